@@ -1,0 +1,203 @@
+/*
+ * lgx — MI355X-native screen-space indirect lighting (C ABI).
+ *
+ * Drop-in for the compute passes of LegitEngine's SSVGI renderer:
+ *   mip pyramid builder      MipBuilder::BuildMips        (src/Render/Common/MipBuilder.h:142-181)
+ *                            + mipLevelBuilder.frag:17-43
+ *   per-mip box pre-blur     BlurBuilder::ApplyBlur       (src/Render/Common/BlurBuilder.h:14-46)
+ *                            + blurLayerBuilder.frag:17-35
+ *   (de)interleave           InterleaveBuilder::Deinterleave / Interleave
+ *                            (src/Render/Common/InterleaveBuilder.h:14-80)
+ *                            + deinterleave.frag:16-28, interleave.frag:15-27
+ *   horizon gather           IndirectLightPass (src/Render/Renderers/SSVGIRenderer.h:223-263)
+ *                            + SSVGI/indirectLighting.frag:114-272
+ *   denoise                  DenoiserPass (SSVGIRenderer.h:265-302) + denoiser.frag:72-185
+ *                            (live regression filter, and the geometry-aware bilateral of
+ *                            its commented weights at denoiser.frag:87-110)
+ *   composite                final gathering pass (SSVGIRenderer.h:304-342) + finalGatherer.frag:42-58
+ *
+ * The reference binds these passes through LegitVulkan's RenderGraph
+ * (AddPass + record funcs, proxy image ids); the C++ RenderGraph in
+ * include/lgx/render_graph.h keeps that shape and calls the entry points
+ * below from its record funcs.
+ *
+ * Conventions
+ *  - All pointers in lgx_image are device pointers (hipMalloc'd memory);
+ *    entry points never allocate device memory.  Host pointers are marked.
+ *  - Streams are hipStream_t (declared here without HIP headers).
+ *  - Return value: 0 on success, a negative LGX_E* code for a rejected
+ *    argument (nothing is launched), or a positive hipError_t from the launch.
+ *    Nothing throws or aborts across the ABI; lgx_strerror names the code.
+ *  - Calls are asynchronous on the given stream unless documented otherwise.
+ */
+#ifndef LGX_H
+#define LGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *lgx_stream_t; /* identical to hipStream_t */
+
+#define LGX_MAX_LEVELS 10 /* MippedProxy::mipsCount (MipBuilder.h:21) */
+#define LGX_MAX_DIRS 256
+#define LGX_MAX_STEPS 1024
+
+/* Pixel formats (vk::Format counterparts in SSVGIRenderer.h:394-404). */
+enum {
+  LGX_R32F = 1,    /* eR32Sfloat */
+  LGX_RG32F = 2,   /* eR32G32Sfloat */
+  LGX_RGBA32F = 3, /* eR32G32B32A32Sfloat */
+  LGX_RG16F = 4,   /* eR16G16Sfloat */
+  LGX_RGBA16F = 5, /* eR16G16B16A16Sfloat */
+  LGX_RGBA8 = 6,   /* eR8G8B8A8Unorm */
+  LGX_RGBA64F = 7, /* FP64 accumulation targets only */
+  LGX_R16F = 8
+};
+
+/* Error codes */
+enum {
+  LGX_OK = 0,
+  LGX_EINVAL = -1,       /* null pointer or bad enum */
+  LGX_EFORMAT = -2,      /* format not accepted by this entry point */
+  LGX_ESIZE = -3,        /* extents inconsistent (e.g. floor-halving, grid divisibility) */
+  LGX_EUNSUPPORTED = -4, /* parameter outside the supported range (dirs, steps, levels) */
+  LGX_EWORKSPACE = -5,   /* workspace missing or not prepared for these parameters */
+  LGX_ENODEVICE = -6     /* no HIP device */
+};
+
+typedef struct {
+  void *ptr;        /* device pointer to texel (0,0) */
+  int32_t w, h;     /* extent in texels */
+  int32_t pitch;    /* row pitch in texels (>= w) */
+  int32_t format;   /* LGX_* format */
+  int32_t reserved; /* must be 0 */
+} lgx_image;
+
+/* G-buffer consumed by the gather (gBufferBuilder.frag:28-38 contract):
+ * depth = radial distance ||P - cam|| (R32F), normal = octahedral world
+ * normal (RG16F or RG32F), albedo (RGBA8 or RGBA32F, linear),
+ * radiance = direct light (RGBA16F or RGBA32F). */
+typedef struct {
+  lgx_image depth, normal, albedo, radiance;
+} lgx_gbuffer;
+
+/* Pyramids sampled by the gather: level 0 is full resolution. */
+typedef struct {
+  int32_t n_levels;
+  lgx_image depth[LGX_MAX_LEVELS];    /* R32F: max-Z Hi-Z, or avg + box pre-blur (reference mode) */
+  lgx_image radiance[LGX_MAX_LEVELS]; /* RGBA16F / RGBA32F avg + box pre-blur */
+} lgx_pyramid;
+
+enum { LGX_MIP_AVG = 0, LGX_MIP_MAX = 1 };                         /* MipBuilder::FilterTypes (MipBuilder.h:136-140) */
+enum { LGX_SAMPLER_TRILINEAR = 0, LGX_SAMPLER_HIZ_POINT = 1 };    /* depth/radiance fetch mode */
+enum { LGX_PREC_F32 = 0, LGX_PREC_F64 = 1 };
+enum { LGX_OUT_INTERLEAVED = 0, LGX_OUT_SLICES = 1 };
+
+typedef struct {
+  float view[16]; /* world -> camera, column-major (IndirectLightingData, SSVGIRenderer.h:476-481) */
+  float proj[16]; /* camera -> ndc, column-major */
+  int32_t width, height; /* viewport (viewportSize.xy) */
+  int32_t dirs;      /* directions per pixel (reference: 4) */
+  int32_t max_steps; /* step cap S; <= 0 = reference step rule only */
+  int32_t grid;      /* interleave pattern size (reference: 4) */
+  int32_t sampler;   /* LGX_SAMPLER_* */
+  int32_t precision; /* LGX_PREC_*: F64 computes and accumulates in double (output RGBA64F) */
+  int32_t out_layout;/* LGX_OUT_*: SLICES writes the output deinterleaved (grid x grid slices) */
+  int32_t *step_counts; /* optional device int32[W*H]: on-screen steps per pixel (sum over dirs), NULL = off */
+  int32_t tile;      /* 0 = default/autotuned launch shape, else an lgx_gather_tiles() index */
+  int32_t reserved[2];
+} lgx_gather_params;
+
+/* Workspace for the gather: per-(pattern, direction) screen directions and
+ * per-(pattern, step) offsets and levels, computed on the host in the
+ * shader's own arithmetic.  Depends only on width/height/dirs/max_steps/
+ * grid/precision, not on the camera: prepare once per resolution. */
+size_t lgx_gather_workspace_bytes(const lgx_gather_params *p);
+/* Fills the workspace (device memory of lgx_gather_workspace_bytes).
+ * Synchronous: returns after the upload has completed. */
+int lgx_gather_prepare(const lgx_gather_params *p, void *workspace);
+
+/* IndirectLightPass (SSVGIRenderer.h:223-263; indirectLighting.frag:114-272).
+ * out: RGBA32F (F32) or RGBA64F (F64), width x height. */
+int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+               lgx_image *out, lgx_stream_t stream);
+
+/* MipBuilder::BuildMips (MipBuilder.h:142-181) + mipLevelBuilder.frag.
+ * levels[0] is the source; levels[1..n_levels-1] are written, each
+ * floor(prev/2) in both extents.  AVG: mean of the 2x2 children in the
+ * order (0,0),(1,0),(0,1),(1,1); MAX: max of them (R32F only). Output is
+ * re-rounded to the level's storage format (FP16 round-to-nearest-even). */
+int lgx_mip_build(const lgx_image *levels, int32_t n_levels, int32_t mode, lgx_stream_t stream);
+
+/* BlurBuilder::ApplyBlur (BlurBuilder.h:14-46) + blurLayerBuilder.frag:
+ * box mean over [-r, r)^2, clamp to edge; radius 0 copies.  Batched: n
+ * (src[i], dst[i], radius[i]) triples in one launch (host arrays of
+ * descriptors).  src[i] and dst[i] must have equal extents and formats. */
+int lgx_box_blur(const lgx_image *src, const lgx_image *dst, const int32_t *radius, int32_t n, lgx_stream_t stream);
+
+/* InterleaveBuilder::Deinterleave (InterleaveBuilder.h:14-46): dst pixel d
+ * = src pixel (d % cell) * grid + d / cell, cell = extent / grid. */
+int lgx_deinterleave(const lgx_image *in, lgx_image *out, int32_t gx, int32_t gy, lgx_stream_t stream);
+/* InterleaveBuilder::Interleave (InterleaveBuilder.h:48-80, with the
+ * interleave program that :60-62 meant to bind): dst pixel i = src pixel
+ * (i % grid) * cell + i / grid. */
+int lgx_interleave(const lgx_image *in, lgx_image *out, int32_t gx, int32_t gy, lgx_stream_t stream);
+
+typedef struct {
+  int32_t radius;       /* taps [-r, r)^2; 0 = passthrough (denoiser.frag:82-86) */
+  float depth_sharpness;  /* 10.5 (denoiser.frag:103) */
+  float normal_sharpness; /* 20.0 (denoiser.frag:105) */
+  int32_t reserved;
+} lgx_bilateral_params;
+
+/* Geometry-aware bilateral (DenoiserPass with the weights of
+ * denoiser.frag:87-110).  in/out RGBA32F; g supplies depth and normal. */
+int lgx_bilateral(const lgx_bilateral_params *p, const lgx_image *in, const lgx_gbuffer *g, lgx_image *out,
+                  lgx_stream_t stream);
+
+/* Live DenoiserPass regression filter (denoiser.frag:72-185): per-pixel
+ * least squares colour = a*depth + b over the 4x4 taps [-2,2)^2. */
+int lgx_regression_denoise(const lgx_image *in, const lgx_gbuffer *g, int32_t radius, lgx_image *out,
+                           lgx_stream_t stream);
+
+/* Final gathering (finalGatherer.frag:42-58): direct + indirect * albedo. */
+int lgx_composite(const lgx_gbuffer *g, const lgx_image *indirect, lgx_image *out, lgx_stream_t stream);
+
+/* ---- seeded analytic scene (replaces ShadowPass/GBufferPass/LightPass,
+ * SSVGIRenderer.h:61-205; the producer is not on the hot path) ---- */
+typedef struct {
+  float wall_albedo[6][3];
+  int32_t n_boxes, n_spheres;
+  float box_min[32][3], box_max[32][3], box_albedo[32][3];
+  float sph_center[8][3], sph_radius[8], sph_albedo[8][3];
+  float light_dir[3]; /* unit, towards the light */
+  float light_intensity;
+  float cam_pos[3];
+  float cam_hor, cam_vert; /* Camera (Scene.h:20-34) */
+  float fovy;              /* radians */
+  float znear, zfar;
+} lgx_scene;
+
+/* Host only: scene from a seed (SplitMix64), boxes <= 32, spheres <= 8. */
+int lgx_scene_make(uint64_t seed, int32_t n_boxes, int32_t n_spheres, lgx_scene *out);
+/* Host only: view and projection matrices for the scene camera:
+ * view = inverse(translate(pos) * rotY(hor) * rotX(vert)),
+ * proj = perspectiveRH_NO(fovy, w/h, znear, zfar) * scale(1,-1,-1)
+ * (SSVGIRenderer.h:54-58). */
+int lgx_scene_matrices(const lgx_scene *s, int32_t width, int32_t height, float view[16], float proj[16]);
+/* Render the G-buffer.  *_cpu: host pointers (the g images' ptr fields are
+ * host memory), computed on host threads; *_gpu: device pointers, async. */
+int lgx_scene_render_cpu(const lgx_scene *s, const lgx_gbuffer *g);
+int lgx_scene_render_gpu(const lgx_scene *s, const lgx_gbuffer *g, lgx_stream_t stream);
+
+const char *lgx_strerror(int code);
+int32_t lgx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

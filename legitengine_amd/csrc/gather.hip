@@ -1,0 +1,536 @@
+// Horizon gather: IndirectLightPass (src/Render/Renderers/SSVGIRenderer.h:223-263)
+// running SSVGI/indirectLighting.frag:114-272 as a gfx950 compute kernel.
+//
+// Design (MI355X):
+//  * Thread mapping per 4x4 interleave slice: every workgroup covers a tile
+//    of ONE slice (all pixels with the same pattern index), so a wavefront's
+//    64 lanes share the Hammersley offsets, i.e. the screen directions and
+//    the geometric step offsets.  Those come from per-slice tables in the
+//    workspace (scalar loads), and all lanes march parallel rays 4 px apart:
+//    coherent pyramid accesses and near-uniform trip counts.
+//  * Exact early exit: the side fade (:220-228) is zero for every sample
+//    outside the open screen rectangle, the ray leaves that convex region
+//    once and never re-enters, so every later step contributes exactly 0.
+//  * Per step, the sample's world-space delta is evaluated from per-direction
+//    dot products (the ray through q is affine in the step offset before
+//    normalisation), the horizon test uses a monotone pseudo-angle, and the
+//    atan2 plus double-angle terms run only when the horizon rises.
+//  * Sample positions q = c + dir * off and the texel/level selection use the
+//    shader's exact operation order (no contraction): the tables come from
+//    the host in the shader's float arithmetic, so point fetches pick the same
+//    texels as the scalar restatement.
+//  * XCD-aware block order: the 16 slices of a tile run back to back on one
+//    XCD, neighbouring tiles stay on that XCD's L2.
+#include <math.h>
+
+#include <vector>
+
+#include "lgx_internal.h"
+
+namespace lgx {
+
+template <class T> struct GatherArgs {
+  T inv_vp[16];
+  T cam[3];
+  T e0[3], ex[3], ey[3];  // unnormalised ray through screen point s: e0 + s.x*ex + s.y*ey
+  T W, H, near_step, log_ratio, ambient;
+  int w, h, grid, cells_x, cells_y, D, S_tab;
+  const T *dirs;   // [grid^2][D][2]
+  const T *steps;  // [grid^2][S_tab][2] = (offset, lod)
+  const float *depth;
+  int depth_pitch;
+  const void *normal;
+  int normal_pitch;
+  int n_levels;
+  const float *pd[LGX_MAX_LEVELS];
+  const void *pr[LGX_MAX_LEVELS];
+  int pd_pitch[LGX_MAX_LEVELS], pr_pitch[LGX_MAX_LEVELS];
+  int lw[LGX_MAX_LEVELS], lh[LGX_MAX_LEVELS];
+  void *out;
+  int out_pitch, out_slices;
+  int *step_counts;
+  int tile_w, tile_h, tiles_x, n_blocks;
+};
+
+template <class T> LGX_D T fast_rsqrt(T v);
+template <> LGX_D float fast_rsqrt<float>(float v) { return __builtin_amdgcn_rsqf(v); }
+template <> LGX_D double fast_rsqrt<double>(double v) { return 1.0 / sqrt(v); }
+
+template <class T> LGX_D T fma_(T a, T b, T c) { return fma(a, b, c); }
+
+// Monotone stand-in for atan2(y, x) on (-pi, pi]: values in [-2, 2].
+template <class T> LGX_D T pseudo_angle(T y, T x) {
+  T r = x / (fabs(x) + fabs(y));
+  return y >= T(0) ? T(1) - r : r - T(1);
+}
+
+template <class T> LGX_D Vec3<T> unproject_far(const GatherArgs<T> &a, T sx, T sy) {
+  // indirectLighting.frag:20-25 with screenPos.z = 1 (:241)
+  const T *m = a.inv_vp;
+  T x = sx * T(2) - T(1), y = sy * T(2) - T(1);
+  T vx = m[0] * x + m[4] * y + m[8] * T(1) + m[12] * T(1);
+  T vy = m[1] * x + m[5] * y + m[9] * T(1) + m[13] * T(1);
+  T vz = m[2] * x + m[6] * y + m[10] * T(1) + m[14] * T(1);
+  T vw = m[3] * x + m[7] * y + m[11] * T(1) + m[15] * T(1);
+  return Vec3<T>{vx / vw, vy / vw, vz / vw};
+}
+
+template <int FMT> struct RadLoad {
+  static LGX_D float4 at(const void *base, size_t i) { return Texel<FMT>::load(base, i); }
+};
+
+// Vulkan clamp-to-edge bilinear in one level (u = s*W - 1/2).
+template <class T, int FMT>
+LGX_D void bilinear(const void *base, int pitch, int lw, int lh, T sx, T sy, T *out, int nch) {
+  T u = sx * T(lw) - T(0.5), v = sy * T(lh) - T(0.5);
+  T fu = floor(u), fv = floor(v);
+  T al = u - fu, be = v - fv;
+  int i0 = (int)fu, j0 = (int)fv;
+  int x0 = min(max(i0, 0), lw - 1), x1 = min(max(i0 + 1, 0), lw - 1);
+  int y0 = min(max(j0, 0), lh - 1), y1 = min(max(j0 + 1, 0), lh - 1);
+  float4 p00 = Texel<FMT>::load(base, (size_t)y0 * pitch + x0);
+  float4 p10 = Texel<FMT>::load(base, (size_t)y0 * pitch + x1);
+  float4 p01 = Texel<FMT>::load(base, (size_t)y1 * pitch + x0);
+  float4 p11 = Texel<FMT>::load(base, (size_t)y1 * pitch + x1);
+  const float *a00 = &p00.x, *a10 = &p10.x, *a01 = &p01.x, *a11 = &p11.x;
+  for (int k = 0; k < nch; k++) {
+    T top = T(a00[k]) * (T(1) - al) + T(a10[k]) * al;
+    T bot = T(a01[k]) * (T(1) - al) + T(a11[k]) * al;
+    out[k] = top * (T(1) - be) + bot * be;
+  }
+}
+
+template <class T, int FMT>
+LGX_D void sample_trilinear(const void *const *lv, const int *pitch, const int *lw, const int *lh, int nl, T sx, T sy,
+                            T lod, T *out, int nch) {
+  T q = T(nl - 1);
+  T d = lod;
+  if (!(d > T(0))) d = T(0);
+  if (d > q) d = q;
+  int l0 = (int)floor(d);
+  int l1 = min(l0 + 1, nl - 1);
+  T delta = d - T(l0);
+  T a[4], b[4];
+  bilinear<T, FMT>(lv[l0], pitch[l0], lw[l0], lh[l0], sx, sy, a, nch);
+  if (delta > T(0) && l1 != l0) {
+    bilinear<T, FMT>(lv[l1], pitch[l1], lw[l1], lh[l1], sx, sy, b, nch);
+    for (int k = 0; k < nch; k++) out[k] = a[k] * (T(1) - delta) + b[k] * delta;
+  } else {
+    for (int k = 0; k < nch; k++) out[k] = a[k];
+  }
+}
+
+template <class T> LGX_D int point_level(T lod, int nl) {
+  int l = 0;
+  if (lod > T(0)) l = (int)min(floor(lod), T(nl - 1));
+  return l;
+}
+template <class T> LGX_D size_t point_index(T sx, T sy, int lw, int lh, int pitch) {
+  T u = floor(sx * T(lw)), v = floor(sy * T(lh));
+  int i = (int)min(max(u, T(0)), T(lw - 1));
+  int j = (int)min(max(v, T(0)), T(lh - 1));
+  return (size_t)j * pitch + i;
+}
+
+template <class T, int SAMPLER, int RFMT, int NFMT, bool COUNT>
+__global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
+  const uint32_t b = xcd_remap(blockIdx.x, (uint32_t)a.n_blocks);
+  const int g2 = a.grid * a.grid;
+  const int k = (int)(b % (uint32_t)g2);
+  const int t = (int)(b / (uint32_t)g2);
+  const int kx = k % a.grid, ky = k / a.grid;
+  const int sx = (t % a.tiles_x) * a.tile_w + (int)(threadIdx.x % (uint32_t)a.tile_w);
+  const int sy = (t / a.tiles_x) * a.tile_h + (int)(threadIdx.x / (uint32_t)a.tile_w);
+  const int x = sx * a.grid + kx, y = sy * a.grid + ky;
+  if (sx >= a.cells_x || sy >= a.cells_y || x >= a.w || y >= a.h) return;
+
+  const Vec3<T> cam{a.cam[0], a.cam[1], a.cam[2]};
+  const T cx = T(x) + T(0.5), cy = T(y) + T(0.5);  // gl_FragCoord (:116)
+  const T scx = cx / a.W, scy = cy / a.H;           // :118
+  const T rc = T(a.depth[(size_t)y * a.depth_pitch + x]);
+  const float4 nt = Texel<NFMT>::load(a.normal, (size_t)y * a.normal_pitch + x);
+  const Vec3<T> n = oct_decode<T>(T(nt.x), T(nt.y));  // :133
+  const Vec3<T> rayc = normalize3(unproject_far(a, scx, scy) - cam);
+  const Vec3<T> P = cam + rayc * rc;  // :130 from the radial distance
+  const Vec3<T> eye = normalize3(cam - P);  // :182
+  const Vec3<T> pdir = normalize3(P - cam);
+  const T ambient = a.ambient;
+  // Ray through the sample, unnormalised, at the pixel centre (U) and per
+  // unit of step offset along a direction (V is formed per direction).
+  const Vec3<T> E0{a.e0[0], a.e0[1], a.e0[2]}, EX{a.ex[0], a.ex[1], a.ex[2]}, EY{a.ey[0], a.ey[1], a.ey[2]};
+  const Vec3<T> U = E0 + EX * scx + EY * scy;
+  const T erc = rc * dot3(eye, rayc);
+  const T en = dot3(eye, n);
+
+  const T *dirs = a.dirs + (size_t)k * a.D * 2;
+  const T *steps = a.steps + (size_t)k * a.S_tab * 2;
+  T sum0 = 0, sum1 = 0, sum2 = 0;
+  int step_total = 0;
+
+  for (int d = 0; d < a.D; d++) {
+    const T dx = dirs[2 * d], dy = dirs[2 * d + 1];
+    // :180-183 tangent of the slice through the offset point
+    const T ox = (cx + dx * T(10e-1)) / a.W, oy = (cy + dy * T(10e-1)) / a.H;
+    const Vec3<T> offp = cam + normalize3(unproject_far(a, ox, oy) - cam) * rc;
+    const Vec3<T> tangent = normalize3(normalize3(offp - cam) - pdir);
+    // :193-198 normal-limited initial horizon
+    const Vec3<T> c1 = cross3(tangent, eye);
+    const Vec3<T> dn = cross3(Vec3<T>{-c1.x, -c1.y, -c1.z}, n);
+    const T hy = dot3(dn, tangent), hx = dot3(dn, eye);
+    T max_h = atan2(hy, hx);
+    T pa_max = pseudo_angle(hy, hx);
+    T q2 = hx * hx + hy * hy;
+    T c2max = (hx * hx - hy * hy) / q2, s2max = (T(2) * hx * hy) / q2;
+    const T tn = dot3(tangent, n);
+    // :202-204 path to the screen border
+    T path;
+    {
+      T ix = T(1) / dx, iy = T(1) / dy;
+      T t1 = (T(0) - cx) * ix, t2 = (a.W - cx) * ix, t3 = (T(0) - cy) * iy, t4 = (a.H - cy) * iy;
+      path = fabs(min(max(t1, t2), max(t3, t4)));
+    }
+    // :209 ambient term: h(0, max_h)
+    T h0 = T(0.25) * en * (T(1) - c2max) + T(0.25) * tn * (T(2) * max_h - s2max);
+    T L0 = ambient * h0, L1 = L0, L2 = L0;
+    // :212 step count, capped by the table (S)
+    int iters = (int)(log(path / a.near_step) / a.log_ratio) + 1;
+    iters = min(iters, a.S_tab);
+    // per-direction constants of the fast delta evaluation
+    const Vec3<T> V = EX * (dx / a.W) + EY * (dy / a.H);
+    const T uu = dot3(U, U), uv2 = T(2) * dot3(U, V), vv = dot3(V, V);
+    const T tu = dot3(tangent, U), tv = dot3(tangent, V);
+    const T eu = dot3(eye, U), ev = dot3(eye, V);
+    const T trc = rc * dot3(tangent, rayc);
+    int j = 0;
+    for (; j < iters; j++) {
+      const T off = steps[2 * j], lod = steps[2 * j + 1];
+      const T qx = cx + dx * off, qy = cy + dy * off;  // :218
+      const T sqx = qx / a.W, sqy = qy / a.H;          // :219
+      if (!(sqx > T(0) && sqx < T(1) && sqy > T(0) && sqy < T(1))) break;
+      T r;
+      if (SAMPLER == LGX_SAMPLER_HIZ_POINT) {
+        const int l = point_level(lod, a.n_levels);
+        r = T(a.pd[l][point_index(sqx, sqy, a.lw[l], a.lh[l], a.pd_pitch[l])]);
+      } else {
+        T tmp[4];
+        sample_trilinear<T, LGX_R32F>((const void *const *)a.pd, a.pd_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod,
+                                      tmp, 1);
+        r = tmp[0];
+      }
+      // :241-252 sample delta projected on (eye, tangent)
+      const T len2 = fma_(off, fma_(off, vv, uv2), uu);
+      const T ri = r * fast_rsqrt(len2);
+      const T hyv = ri * fma_(off, tv, tu) - trc;
+      const T hxv = ri * fma_(off, ev, eu) - erc;
+      const T pa = pseudo_angle(hyv, hxv);
+      if (pa < pa_max) {  // :254
+        const T th = atan2(hyv, hxv);
+        const T qq = hxv * hxv + hyv * hyv;
+        const T c2 = (hxv * hxv - hyv * hyv) / qq, s2 = (T(2) * hxv * hyv) / qq;
+        T side = T(1);
+        {
+          const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
+          side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
+          side *= min(max(sqx * inv_width, T(0)), T(1));
+          side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
+          side *= min(max(sqy * inv_width, T(0)), T(1));
+        }
+        const T hc = (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
+        T ls[4];
+        if (SAMPLER == LGX_SAMPLER_HIZ_POINT) {
+          const int l = point_level(lod, a.n_levels);
+          float4 v = Texel<RFMT>::load(a.pr[l], point_index(sqx, sqy, a.lw[l], a.lh[l], a.pr_pitch[l]));
+          ls[0] = T(v.x);
+          ls[1] = T(v.y);
+          ls[2] = T(v.z);
+        } else {
+          sample_trilinear<T, RFMT>((const void *const *)a.pr, a.pr_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod, ls,
+                                    3);
+        }
+        L0 += ls[0] * hc;  // :261
+        L1 += ls[1] * hc;
+        L2 += ls[2] * hc;
+        L0 -= ambient * hc;  // :262
+        L1 -= ambient * hc;
+        L2 -= ambient * hc;
+        max_h = th;  // :263
+        pa_max = pa;
+        c2max = c2;
+        s2max = s2;
+      }
+    }
+    if (COUNT) step_total += j;
+    sum0 += T(2) * L0 / T(a.D);  // :268
+    sum1 += T(2) * L1 / T(a.D);
+    sum2 += T(2) * L2 / T(a.D);
+  }
+
+  int ox = x, oy = y;
+  if (a.out_slices) {
+    ox = kx * a.cells_x + sx;
+    oy = ky * a.cells_y + sy;
+  }
+  const size_t o = (size_t)oy * a.out_pitch + ox;
+  if (sizeof(T) == 4) {
+    ((float4 *)a.out)[o] = make_float4((float)sum0, (float)sum1, (float)sum2, 1.0f);  // :270
+  } else {
+    double *po = (double *)a.out + o * 4;
+    po[0] = (double)sum0;
+    po[1] = (double)sum1;
+    po[2] = (double)sum2;
+    po[3] = 1.0;
+  }
+  if (COUNT) a.step_counts[(size_t)y * a.w + x] = step_total;
+}
+
+// ---------------------------------------------------------------- host side
+
+struct GatherTableShape {
+  int g2, D, S_tab;
+  size_t elem;
+};
+
+static void hammersley(int i, int n, float &aa, float &bb) {
+  uint32_t v = ((uint32_t)i << 16u) | ((uint32_t)i >> 16u);
+  v = ((v & 0x55555555u) << 1u) | ((v & 0xAAAAAAAAu) >> 1u);
+  v = ((v & 0x33333333u) << 2u) | ((v & 0xCCCCCCCCu) >> 2u);
+  v = ((v & 0x0F0F0F0Fu) << 4u) | ((v & 0xF0F0F0F0u) >> 4u);
+  v = ((v & 0x00FF00FFu) << 8u) | ((v & 0xFF00FF00u) >> 8u);
+  aa = (float)i / (float)n;
+  bb = (float)v / (float)0xffffffffu;
+}
+
+template <class T> static int table_steps(const lgx_gather_params *p) {
+  const T pi = T(3.1415f);
+  const T W = T(p->width), H = T(p->height);
+  const T near_step = W / T(1000);
+  const T ratio = T(2) * pi / T(p->dirs) + T(1);
+  const T diag = std::sqrt(W * W + H * H);
+  long it = (long)(std::log(diag / near_step) / std::log(ratio)) + 2;
+  if (p->max_steps > 0 && it > p->max_steps) it = p->max_steps;
+  if (it < 1) it = 1;
+  return (int)it;
+}
+
+static bool gather_params_ok(const lgx_gather_params *p) {
+  return p && p->width > 0 && p->height > 0 && p->dirs >= 1 && p->dirs <= LGX_MAX_DIRS && p->grid >= 1 &&
+         p->grid <= 8 && (p->sampler == LGX_SAMPLER_TRILINEAR || p->sampler == LGX_SAMPLER_HIZ_POINT) &&
+         (p->precision == LGX_PREC_F32 || p->precision == LGX_PREC_F64) &&
+         (p->out_layout == LGX_OUT_INTERLEAVED || p->out_layout == LGX_OUT_SLICES);
+}
+
+static GatherTableShape table_shape(const lgx_gather_params *p) {
+  GatherTableShape s;
+  s.g2 = p->grid * p->grid;
+  s.D = p->dirs;
+  s.S_tab = p->precision == LGX_PREC_F64 ? table_steps<double>(p) : table_steps<float>(p);
+  s.elem = p->precision == LGX_PREC_F64 ? sizeof(double) : sizeof(float);
+  return s;
+}
+
+// Tables in the shader's own arithmetic and evaluation order
+// (indirectLighting.frag:161-177 directions, :217 offsets, :234 lod).
+template <class T> static void fill_tables(const lgx_gather_params *p, const GatherTableShape &s, std::vector<T> &out) {
+  const T pi = T(3.1415f);
+  const T W = T(p->width);
+  const T near_step = W / T(1000);
+  const int D = p->dirs;
+  out.assign((size_t)s.g2 * D * 2 + (size_t)s.g2 * s.S_tab * 2, T(0));
+  T *dirs = out.data();
+  T *steps = dirs + (size_t)s.g2 * D * 2;
+  for (int k = 0; k < s.g2; k++) {
+    float angf, linf;
+    hammersley(k, s.g2, angf, linf);
+    const T ang_offset = T(angf), lin_offset = T(linf);
+    const T pixel_ang_offset = T(2) * pi / T(D) * ang_offset;
+    for (int d = 0; d < D; d++) {
+      T screen_ang = pixel_ang_offset + T(2) * pi / T(D) * T(d);
+      dirs[((size_t)k * D + d) * 2 + 0] = std::cos(screen_ang);
+      dirs[((size_t)k * D + d) * 2 + 1] = std::sin(screen_ang);
+    }
+    const T ratio = T(2) * pi / T(D) + T(1);
+    for (int j = 0; j < s.S_tab; j++) {
+      T off = near_step * std::pow(ratio, T(j) + lin_offset) + T(1) - near_step;
+      T lod = std::log(std::max(T(0), T(2) * pi / T(D) * (off - T(1)) * T(0.5f))) / std::log(T(2)) + T(-2);
+      steps[((size_t)k * s.S_tab + j) * 2 + 0] = off;
+      steps[((size_t)k * s.S_tab + j) * 2 + 1] = lod;
+    }
+  }
+}
+
+template <class T> static void frame_constants(const lgx_gather_params *p, GatherArgs<T> &a) {
+  Mat4 view, proj;
+  memcpy(view.m, p->view, 64);
+  memcpy(proj.m, p->proj, 64);
+  double invvp[16], cam[3];
+  if (sizeof(T) == 4) {
+    Mat4 vp = mat4_mul(proj, view);
+    Mat4 inv = mat4_inverse(vp);
+    Mat4 iv = mat4_inverse(view);
+    for (int i = 0; i < 16; i++) invvp[i] = inv.m[i];
+    for (int i = 0; i < 3; i++) cam[i] = iv.m[12 + i];  // invV * (0,0,0,1)
+  } else {
+    Mat4d vd, pd;
+    for (int i = 0; i < 16; i++) {
+      vd.m[i] = p->view[i];
+      pd.m[i] = p->proj[i];
+    }
+    Mat4d inv = mat4d_inverse(mat4d_mul(pd, vd));
+    Mat4d iv = mat4d_inverse(vd);
+    for (int i = 0; i < 16; i++) invvp[i] = inv.m[i];
+    for (int i = 0; i < 3; i++) cam[i] = iv.m[12 + i];
+  }
+  for (int i = 0; i < 16; i++) a.inv_vp[i] = T(invvp[i]);
+  for (int i = 0; i < 3; i++) a.cam[i] = T(cam[i]);
+  // v(s) = invVP * (2s-1, 1, 1) = c0*(2sx-1) + c1*(2sy-1) + c2 + c3;
+  // ray(s) ~ sign(v.w) * (v.xyz - cam * v.w), affine in s.
+  double base[4], c0[4], c1[4];
+  for (int r = 0; r < 4; r++) {
+    base[r] = invvp[8 + r] + invvp[12 + r] - invvp[0 + r] - invvp[4 + r];
+    c0[r] = 2.0 * invvp[0 + r];
+    c1[r] = 2.0 * invvp[4 + r];
+  }
+  double wc = base[3] + 0.5 * c0[3] + 0.5 * c1[3];
+  double sg = wc < 0 ? -1.0 : 1.0;
+  for (int i = 0; i < 3; i++) {
+    a.e0[i] = T(sg * (base[i] - cam[i] * base[3]));
+    a.ex[i] = T(sg * (c0[i] - cam[i] * c0[3]));
+    a.ey[i] = T(sg * (c1[i] - cam[i] * c1[3]));
+  }
+}
+
+template <class T, int SAMPLER, int RFMT, int NFMT>
+static hipError_t launch_gather(const GatherArgs<T> &a, bool count, hipStream_t s) {
+  dim3 grid((unsigned)a.n_blocks), block(256);
+  if (count)
+    hipLaunchKernelGGL((gather_kernel<T, SAMPLER, RFMT, NFMT, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((gather_kernel<T, SAMPLER, RFMT, NFMT, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <class T, int SAMPLER>
+static hipError_t dispatch_fmt(const GatherArgs<T> &a, int rfmt, int nfmt, bool count, hipStream_t s) {
+  if (rfmt == LGX_RGBA16F && nfmt == LGX_RG16F) return launch_gather<T, SAMPLER, LGX_RGBA16F, LGX_RG16F>(a, count, s);
+  if (rfmt == LGX_RGBA32F && nfmt == LGX_RG32F) return launch_gather<T, SAMPLER, LGX_RGBA32F, LGX_RG32F>(a, count, s);
+  if (rfmt == LGX_RGBA16F && nfmt == LGX_RG32F) return launch_gather<T, SAMPLER, LGX_RGBA16F, LGX_RG32F>(a, count, s);
+  if (rfmt == LGX_RGBA32F && nfmt == LGX_RG16F) return launch_gather<T, SAMPLER, LGX_RGBA32F, LGX_RG16F>(a, count, s);
+  return hipErrorInvalidValue;
+}
+
+// Launch shapes offered to the autotuner: tile_w x tile_h slice pixels per
+// 256-thread workgroup.
+static const int kTiles[][2] = {{16, 16}, {8, 32}, {32, 8}, {64, 4}};
+static const int kNumTiles = 4;
+
+}  // namespace lgx
+
+using namespace lgx;
+
+extern "C" size_t lgx_gather_workspace_bytes(const lgx_gather_params *p) {
+  if (!gather_params_ok(p)) return 0;
+  GatherTableShape s = table_shape(p);
+  return ((size_t)s.g2 * s.D * 2 + (size_t)s.g2 * s.S_tab * 2) * s.elem;
+}
+
+extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
+  if (!gather_params_ok(p) || !workspace) return LGX_EINVAL;
+  GatherTableShape s = table_shape(p);
+  if (s.S_tab > LGX_MAX_STEPS) return LGX_EUNSUPPORTED;
+  hipError_t e;
+  if (p->precision == LGX_PREC_F64) {
+    std::vector<double> t;
+    fill_tables<double>(p, s, t);
+    e = hipMemcpy(workspace, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
+  } else {
+    std::vector<float> t;
+    fill_tables<float>(p, s, t);
+    e = hipMemcpy(workspace, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+  }
+  return hip_status(e);
+}
+
+extern "C" int lgx_gather_tiles(void) { return kNumTiles; }
+
+template <class T>
+static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+                       lgx_image *out, hipStream_t stream) {
+  GatherTableShape s = table_shape(p);
+  GatherArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  frame_constants<T>(p, a);
+  const T pi = T(3.1415f);
+  a.W = T(p->width);
+  a.H = T(p->height);
+  a.near_step = a.W / T(1000);
+  a.log_ratio = std::log(T(2) * pi / T(p->dirs) + T(1));
+  a.ambient = T(0.01f);
+  a.w = p->width;
+  a.h = p->height;
+  a.grid = p->grid;
+  a.cells_x = (p->width + p->grid - 1) / p->grid;
+  a.cells_y = (p->height + p->grid - 1) / p->grid;
+  a.D = p->dirs;
+  a.S_tab = s.S_tab;
+  a.dirs = (const T *)ws;
+  a.steps = (const T *)ws + (size_t)s.g2 * s.D * 2;
+  a.depth = (const float *)g->depth.ptr;
+  a.depth_pitch = g->depth.pitch;
+  a.normal = g->normal.ptr;
+  a.normal_pitch = g->normal.pitch;
+  a.n_levels = pyr->n_levels;
+  for (int l = 0; l < pyr->n_levels; l++) {
+    a.pd[l] = (const float *)pyr->depth[l].ptr;
+    a.pd_pitch[l] = pyr->depth[l].pitch;
+    a.pr[l] = pyr->radiance[l].ptr;
+    a.pr_pitch[l] = pyr->radiance[l].pitch;
+    a.lw[l] = pyr->depth[l].w;
+    a.lh[l] = pyr->depth[l].h;
+  }
+  a.out = out->ptr;
+  a.out_pitch = out->pitch;
+  a.out_slices = p->out_layout == LGX_OUT_SLICES;
+  a.step_counts = p->step_counts;
+  int tile = p->tile;
+  if (tile < 0 || tile >= kNumTiles) return LGX_EUNSUPPORTED;
+  a.tile_w = kTiles[tile][0];
+  a.tile_h = kTiles[tile][1];
+  a.tiles_x = (a.cells_x + a.tile_w - 1) / a.tile_w;
+  int tiles_y = (a.cells_y + a.tile_h - 1) / a.tile_h;
+  a.n_blocks = a.tiles_x * tiles_y * s.g2;
+  hipError_t e;
+  const int rf = pyr->radiance[0].format, nf = g->normal.format;
+  if (p->sampler == LGX_SAMPLER_HIZ_POINT)
+    e = dispatch_fmt<T, LGX_SAMPLER_HIZ_POINT>(a, rf, nf, p->step_counts != nullptr, stream);
+  else
+    e = dispatch_fmt<T, LGX_SAMPLER_TRILINEAR>(a, rf, nf, p->step_counts != nullptr, stream);
+  return hip_status(e);
+}
+
+extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
+                          const lgx_pyramid *pyr, lgx_image *out, lgx_stream_t stream) {
+  if (!gather_params_ok(p) || !g || !pyr || !out) return LGX_EINVAL;
+  if (!workspace) return LGX_EWORKSPACE;
+  if (!image_ok(&g->depth) || !image_ok(&g->normal) || !image_ok(out)) return LGX_EINVAL;
+  if (g->depth.format != LGX_R32F) return LGX_EFORMAT;
+  if (g->normal.format != LGX_RG16F && g->normal.format != LGX_RG32F) return LGX_EFORMAT;
+  if (g->depth.w != p->width || g->depth.h != p->height || g->normal.w != p->width || g->normal.h != p->height)
+    return LGX_ESIZE;
+  if (out->w != p->width || out->h != p->height) return LGX_ESIZE;
+  if (out->format != (p->precision == LGX_PREC_F64 ? LGX_RGBA64F : LGX_RGBA32F)) return LGX_EFORMAT;
+  if (p->out_layout == LGX_OUT_SLICES && (p->width % p->grid || p->height % p->grid)) return LGX_ESIZE;
+  if (pyr->n_levels < 1 || pyr->n_levels > LGX_MAX_LEVELS) return LGX_EUNSUPPORTED;
+  const int rf = pyr->radiance[0].format;
+  if (rf != LGX_RGBA16F && rf != LGX_RGBA32F) return LGX_EFORMAT;
+  int ew = p->width, eh = p->height;
+  for (int l = 0; l < pyr->n_levels; l++) {
+    const lgx_image &d = pyr->depth[l], &r = pyr->radiance[l];
+    if (!image_ok(&d) || !image_ok(&r)) return LGX_EINVAL;
+    if (d.format != LGX_R32F || r.format != rf) return LGX_EFORMAT;
+    if (d.w != ew || d.h != eh || r.w != ew || r.h != eh) return LGX_ESIZE;  // floor halving (MipBuilder.h:148-152)
+    ew /= 2;
+    eh /= 2;
+  }
+  if (p->precision == LGX_PREC_F64) return gather_impl<double>(p, workspace, g, pyr, out, (hipStream_t)stream);
+  return gather_impl<float>(p, workspace, g, pyr, out, (hipStream_t)stream);
+}

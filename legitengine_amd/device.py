@@ -1,0 +1,177 @@
+"""Device-memory plumbing for the lgx C ABI: torch tensors as HIP buffers.
+
+Torch supplies allocations and the current stream; every pass runs in a
+HIP kernel behind liblgx.so.  Raises when no HIP device is present.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import lgx
+
+_TORCH_DTYPE = {
+    lgx.R32F: (torch.float32, 1), lgx.RG32F: (torch.float32, 2), lgx.RGBA32F: (torch.float32, 4),
+    lgx.RG16F: (torch.float16, 2), lgx.RGBA16F: (torch.float16, 4), lgx.RGBA8: (torch.uint8, 4),
+    lgx.RGBA64F: (torch.float64, 4), lgx.R16F: (torch.float16, 1),
+}
+
+
+def require_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device: the lgx passes run only on the GPU")
+    lgx.lib()
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class DImage:
+    """A device image: torch storage + its lgx_image descriptor."""
+
+    def __init__(self, w, h, fmt, tensor=None, device="cuda"):
+        dt, c = _TORCH_DTYPE[fmt]
+        shape = (h, w) if c == 1 else (h, w, c)
+        self.t = tensor if tensor is not None else torch.empty(shape, dtype=dt, device=device)
+        assert tuple(self.t.shape) == shape and self.t.dtype == dt and self.t.is_contiguous()
+        self.w, self.h, self.fmt = w, h, fmt
+
+    @property
+    def desc(self):
+        return lgx.Image(C.c_void_p(self.t.data_ptr()), self.w, self.h, self.w, self.fmt, 0)
+
+    @staticmethod
+    def from_numpy(a, fmt):
+        dt, c = _TORCH_DTYPE[fmt]
+        t = torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+        h, w = a.shape[0], a.shape[1]
+        if c == 1:
+            t = t.reshape(h, w)
+        return DImage(w, h, fmt, tensor=t.to(dt).contiguous())
+
+    def numpy(self):
+        return self.t.cpu().numpy()
+
+
+class GBufferDev:
+    def __init__(self, depth, normal, albedo, radiance):
+        self.depth, self.normal, self.albedo, self.radiance = depth, normal, albedo, radiance
+        self.w, self.h = depth.w, depth.h
+
+    @property
+    def desc(self):
+        return lgx.GBuffer(self.depth.desc, self.normal.desc, self.albedo.desc, self.radiance.desc)
+
+    @staticmethod
+    def from_host(gb):
+        f = gb["formats"]
+        return GBufferDev(DImage.from_numpy(gb["depth"], lgx.R32F), DImage.from_numpy(gb["normal"], f["normal"]),
+                          DImage.from_numpy(gb["albedo"], f["albedo"]),
+                          DImage.from_numpy(gb["radiance"], f["radiance"]))
+
+    @staticmethod
+    def alloc(w, h, kind="mixed"):
+        f = lgx.GBUF_FORMATS[kind]
+        return GBufferDev(DImage(w, h, lgx.R32F), DImage(w, h, f["normal"]), DImage(w, h, f["albedo"]),
+                          DImage(w, h, f["radiance"]))
+
+    def render(self, scene, stream=None):
+        g = self.desc
+        lgx.check(lgx.lib().lgx_scene_render_gpu(C.byref(scene), C.byref(g), stream_handle(stream)),
+                  "lgx_scene_render_gpu")
+        return self
+
+
+class PyramidDev:
+    """Depth (R32F) and radiance pyramids plus their pre-blurred copies."""
+
+    def __init__(self, gb, n_levels, sampler):
+        self.n, self.sampler = n_levels, sampler
+        sizes = lgx.level_sizes(gb.w, gb.h, n_levels)
+        rf = gb.radiance.fmt
+        self.rad = [gb.radiance] + [DImage(w, h, rf) for (w, h) in sizes[1:]]
+        self.rad_b = [gb.radiance] + [DImage(w, h, rf) for (w, h) in sizes[1:]]  # r=0 at level 0 is a copy
+        self.dep = [gb.depth] + [DImage(w, h, lgx.R32F) for (w, h) in sizes[1:]]
+        if sampler == lgx.SAMPLER_TRILINEAR:
+            self.dep_b = [gb.depth] + [DImage(w, h, lgx.R32F) for (w, h) in sizes[1:]]
+        else:
+            self.dep_b = self.dep  # max-Z Hi-Z is not blurred
+
+    def build(self, stream=None):
+        L, s = lgx.lib(), stream_handle(stream)
+        n = self.n
+        if n > 1:
+            rl = (lgx.Image * n)(*[d.desc for d in self.rad])
+            lgx.check(L.lgx_mip_build(rl, n, lgx.MIP_AVG, s), "lgx_mip_build radiance")
+            dl = (lgx.Image * n)(*[d.desc for d in self.dep])
+            mode = lgx.MIP_MAX if self.sampler == lgx.SAMPLER_HIZ_POINT else lgx.MIP_AVG
+            lgx.check(L.lgx_mip_build(dl, n, mode, s), "lgx_mip_build depth")
+            src = [d.desc for d in self.rad[1:]]
+            dst = [d.desc for d in self.rad_b[1:]]
+            if self.sampler == lgx.SAMPLER_TRILINEAR:
+                src_d = [d.desc for d in self.dep[1:]]
+                dst_d = [d.desc for d in self.dep_b[1:]]
+                lgx.check(L.lgx_box_blur((lgx.Image * (n - 1))(*src_d), (lgx.Image * (n - 1))(*dst_d),
+                                         (C.c_int32 * (n - 1))(*([2] * (n - 1))), n - 1, s), "lgx_box_blur depth")
+            lgx.check(L.lgx_box_blur((lgx.Image * (n - 1))(*src), (lgx.Image * (n - 1))(*dst),
+                                     (C.c_int32 * (n - 1))(*([2] * (n - 1))), n - 1, s), "lgx_box_blur radiance")
+        return self
+
+    @property
+    def desc(self):
+        p = lgx.Pyramid()
+        p.n_levels = self.n
+        for l in range(self.n):
+            p.depth[l] = self.dep_b[l].desc
+            p.radiance[l] = self.rad_b[l].desc
+        return p
+
+
+class Gather:
+    """Prepared gather (workspace tables for one resolution / parameter set)."""
+
+    def __init__(self, w, h, dirs, max_steps, sampler, grid=4, precision=lgx.PREC_F32, out_layout=lgx.OUT_INTERLEAVED,
+                 tile=0):
+        p = lgx.GatherParams()
+        p.width, p.height, p.dirs, p.max_steps, p.grid = w, h, dirs, max_steps, grid
+        p.sampler, p.precision, p.out_layout, p.tile = sampler, precision, out_layout, tile
+        self.p = p
+        nbytes = lgx.lib().lgx_gather_workspace_bytes(C.byref(p))
+        if nbytes == 0:
+            raise lgx.LgxError(lgx.EINVAL, "lgx_gather_workspace_bytes")
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        lgx.check(lgx.lib().lgx_gather_prepare(C.byref(p), C.c_void_p(self.ws.data_ptr())), "lgx_gather_prepare")
+        self.out_fmt = lgx.RGBA64F if precision == lgx.PREC_F64 else lgx.RGBA32F
+
+    def __call__(self, view, proj, gb, pyr, out=None, step_counts=None, stream=None):
+        p = self.p
+        p.view[:] = [float(v) for v in view]
+        p.proj[:] = [float(v) for v in proj]
+        p.step_counts = C.c_void_p(step_counts.data_ptr()) if step_counts is not None else None
+        if out is None:
+            out = DImage(p.width, p.height, self.out_fmt)
+        g, pd, o = gb.desc, pyr.desc, out.desc
+        lgx.check(lgx.lib().lgx_gather(C.byref(p), C.c_void_p(self.ws.data_ptr()), C.byref(g), C.byref(pd),
+                                       C.byref(o), stream_handle(stream)), "lgx_gather")
+        return out
+
+
+def bilateral(inp, gb, out=None, radius=4, ds=10.5, ns=20.0, stream=None):
+    if out is None:
+        out = DImage(inp.w, inp.h, lgx.RGBA32F)
+    bp = lgx.BilateralParams(radius, ds, ns, 0)
+    i, g, o = inp.desc, gb.desc, out.desc
+    lgx.check(lgx.lib().lgx_bilateral(C.byref(bp), C.byref(i), C.byref(g), C.byref(o), stream_handle(stream)),
+              "lgx_bilateral")
+    return out
+
+
+def permute(inp, out=None, inverse=False, gx=4, gy=4, stream=None):
+    if out is None:
+        out = DImage(inp.w, inp.h, inp.fmt)
+    i, o = inp.desc, out.desc
+    fn = lgx.lib().lgx_interleave if inverse else lgx.lib().lgx_deinterleave
+    lgx.check(fn(C.byref(i), C.byref(o), gx, gy, stream_handle(stream)), "lgx_(de)interleave")
+    return out

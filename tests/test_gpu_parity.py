@@ -1,0 +1,264 @@
+"""Parity of the HIP passes (through the lgx C ABI) with the CPU oracle.
+
+Bars: permutations, Hi-Z max, avg mips and box pre-blur bit-exact; gather
+FP32 irradiance within 1e-4 relative L2 of the FP32 oracle (and its
+per-pixel step counts equal); bilateral within 1e-5 (expf ulps only); the
+FP64 gather within 1e-9 of the FP64 oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from legitengine_amd import lgx  # noqa: E402
+from oracle import binding as orc  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+GATHER_TOL = 1e-4   # FP32 irradiance vs FP32 oracle, relative L2
+BILAT_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from legitengine_amd import device
+    device.require_device()
+    return device
+
+
+def host_scene(seed, w, h, kind, boxes, spheres):
+    sc = lgx.make_scene(seed, boxes, spheres)
+    view, proj = lgx.scene_matrices(sc, w, h)
+    return sc, view, proj, lgx.render_gbuffer_cpu(sc, w, h, kind)
+
+
+@pytest.fixture(scope="module")
+def c1(dev):
+    """Config 1: 256x256, FP32 G-buffer, 8 boxes, 2 spheres."""
+    return [host_scene(s, 256, 256, "f32", 8, 2) for s in (0, 1)]
+
+
+@pytest.fixture(scope="module")
+def c2(dev):
+    """Config 2: 1920x1080, depth FP32, normals/radiance FP16, albedo RGBA8, 32 boxes, 8 spheres."""
+    return [host_scene(100, 1920, 1080, "mixed", 32, 8)]
+
+
+def test_scene_gpu_matches_cpu(dev):
+    sc = lgx.make_scene(7, 8, 2)
+    w, h = 320, 192
+    host = lgx.render_gbuffer_cpu(sc, w, h, "mixed")
+    g = dev.GBufferDev.alloc(w, h, "mixed").render(sc)
+    torch.cuda.synchronize()
+    d = g.depth.numpy()
+    assert np.allclose(d, host["depth"], rtol=1e-6, atol=1e-6)
+    same = (g.radiance.numpy() == host["radiance"]).mean()
+    assert same > 0.999
+
+
+@pytest.mark.parametrize("w,h,fmt", [(256, 256, lgx.RGBA32F), (1920, 1080, lgx.RGBA16F), (64, 32, lgx.R32F),
+                                     (1280, 720, lgx.RG16F), (48, 16, lgx.RGBA64F)])
+def test_interleave_bit_exact(dev, w, h, fmt):
+    rng = np.random.default_rng(w * h + fmt)
+    a = dev.DImage(w, h, fmt)
+    nbytes = a.t.numel() * a.t.element_size()
+    raw = rng.integers(0, 256, nbytes, dtype=np.uint8)
+    a.t.view(torch.uint8).reshape(-1).copy_(torch.from_numpy(raw))
+    host = a.numpy()
+    d = dev.permute(a, inverse=False)
+    i = dev.permute(d, inverse=True)
+    torch.cuda.synchronize()
+    ref_d = orc.deinterleave(host)
+    assert d.numpy().tobytes() == ref_d.tobytes()
+    assert i.numpy().tobytes() == host.tobytes()  # Interleave o Deinterleave = id
+    assert orc.interleave(ref_d).tobytes() == host.tobytes()
+
+
+def _pyr_host(gb, n, sampler):
+    depth = np.ascontiguousarray(gb["depth"], np.float32)
+    rad = orc.widen(gb["radiance"])
+    return orc.build_pyramids(depth, rad, n, sampler, gb["radiance"].dtype == np.float16)
+
+
+@pytest.mark.parametrize("cfg,n", [("c1", 5), ("c2", 6)])
+@pytest.mark.parametrize("sampler", [lgx.SAMPLER_TRILINEAR, lgx.SAMPLER_HIZ_POINT])
+def test_pyramids_bit_exact(dev, request, cfg, n, sampler):
+    sc, view, proj, gb = request.getfixturevalue(cfg)[0]
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, n, sampler).build()
+    torch.cuda.synchronize()
+    dep_b, rad_b, dep, rad = _pyr_host(gb, n, sampler)
+    for l in range(1, n):
+        gd = pyr.dep[l].numpy().astype(np.float32)
+        assert gd.tobytes() == dep[l].astype(np.float32).tobytes(), f"depth mip {l}"
+        gr = pyr.rad[l].numpy().astype(np.float32)
+        assert gr.tobytes() == rad[l].astype(np.float32).tobytes(), f"radiance mip {l}"
+        gb_ = pyr.rad_b[l].numpy().astype(np.float32)
+        assert gb_.tobytes() == rad_b[l].astype(np.float32).tobytes(), f"blurred radiance {l}"
+        if sampler == lgx.SAMPLER_TRILINEAR:
+            assert pyr.dep_b[l].numpy().tobytes() == dep_b[l].astype(np.float32).tobytes(), f"blurred depth {l}"
+    if sampler == lgx.SAMPLER_HIZ_POINT:  # max-Z invariant: parent >= each child
+        for l in range(1, n):
+            p = pyr.dep[l].numpy()
+            c = pyr.dep[l - 1].numpy()[: 2 * p.shape[0], : 2 * p.shape[1]]
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    assert (p >= c[dy::2, dx::2]).all()
+
+
+def _gpu_gather(dev, gb, view, proj, n, dirs, steps, sampler, precision=lgx.PREC_F32):
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, n, sampler).build()
+    ga = dev.Gather(g.w, g.h, dirs, steps, sampler, precision=precision)
+    counts = torch.zeros((g.h, g.w), dtype=torch.int32, device="cuda")
+    out = ga(view, proj, g, pyr, step_counts=counts)
+    torch.cuda.synchronize()
+    return g, pyr, out.numpy(), counts.cpu().numpy()
+
+
+@pytest.mark.parametrize("seed_idx", [0, 1])
+@pytest.mark.parametrize("sampler", [lgx.SAMPLER_TRILINEAR, lgx.SAMPLER_HIZ_POINT])
+@pytest.mark.parametrize("dirs,steps", [(16, 32), (4, 0)])
+def test_gather_c1(dev, c1, seed_idx, sampler, dirs, steps):
+    sc, view, proj, gb = c1[seed_idx]
+    g, pyr, out, counts = _gpu_gather(dev, gb, view, proj, 5, dirs, steps, sampler)
+    dep_b, rad_b, _, _ = _pyr_host(gb, 5, sampler)
+    ref, ref_steps = orc.gather(view, proj, gb["depth"], orc.widen(gb["normal"]), dep_b, rad_b, dirs, steps,
+                                sampler, with_steps=True)
+    err = orc.rel_l2(out, ref)
+    assert err <= GATHER_TOL, err
+    assert (out[..., 3] == 1.0).all()
+    assert np.array_equal(counts, ref_steps)
+
+
+@pytest.mark.parametrize("sampler", [lgx.SAMPLER_HIZ_POINT, lgx.SAMPLER_TRILINEAR])
+def test_gather_c2_crops(dev, c2, sampler):
+    sc, view, proj, gb = c2[0]
+    g, pyr, out, counts = _gpu_gather(dev, gb, view, proj, 6, 32, 64, sampler)
+    dep_b = [d.numpy().astype(np.float32) for d in pyr.dep_b]
+    rad_b = [orc.widen(r.numpy()) for r in pyr.rad_b]
+    octn = orc.widen(gb["normal"])
+    num = den = 0.0
+    for (x0, y0) in [(0, 0), (928, 508), (1856, 1016), (400, 900)]:
+        ref, ref_steps = orc.gather(view, proj, gb["depth"], octn, dep_b, rad_b, 32, 64, sampler,
+                                    crop=(x0, y0, 64, 64), with_steps=True)
+        o = out[y0:y0 + 64, x0:x0 + 64]
+        num += ((o[..., :3].astype(np.float64) - ref[..., :3]) ** 2).sum()
+        den += (ref[..., :3].astype(np.float64) ** 2).sum()
+        assert np.array_equal(counts[y0:y0 + 64, x0:x0 + 64], ref_steps)
+    err = np.sqrt(num / den)
+    assert err <= GATHER_TOL, err
+
+
+def test_gather_f64_crop(dev, c1):
+    sc, view, proj, gb = c1[0]
+    g, pyr, out, counts = _gpu_gather(dev, gb, view, proj, 5, 16, 32, lgx.SAMPLER_HIZ_POINT, lgx.PREC_F64)
+    dep_b, rad_b, _, _ = _pyr_host(gb, 5, lgx.SAMPLER_HIZ_POINT)
+    ref = orc.gather(view, proj, gb["depth"], orc.widen(gb["normal"]), dep_b, rad_b, 16, 32, 1, precision="f64",
+                     crop=(96, 96, 64, 64))
+    err = orc.rel_l2(out[96:160, 96:160], ref)
+    assert err <= 1e-9, err
+
+
+def test_gather_slices_layout(dev, c1):
+    sc, view, proj, gb = c1[0]
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, 5, lgx.SAMPLER_HIZ_POINT).build()
+    a = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT)(view, proj, g, pyr)
+    b = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT, out_layout=lgx.OUT_SLICES)(view, proj, g, pyr)
+    back = dev.permute(b, inverse=True)
+    torch.cuda.synchronize()
+    assert back.numpy().tobytes() == a.numpy().tobytes()
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+def test_gather_tiles_identical(dev, c1, tile):
+    sc, view, proj, gb = c1[1]
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, 5, lgx.SAMPLER_HIZ_POINT).build()
+    a = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT, tile=0)(view, proj, g, pyr)
+    b = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT, tile=tile)(view, proj, g, pyr)
+    torch.cuda.synchronize()
+    assert a.numpy().tobytes() == b.numpy().tobytes()
+
+
+@pytest.mark.parametrize("kind,w,h", [("f32", 256, 256), ("mixed", 480, 272)])
+def test_bilateral(dev, kind, w, h):
+    sc, view, proj, gb = host_scene(3, w, h, kind, 8, 2)
+    g = dev.GBufferDev.from_host(gb)
+    rng = np.random.default_rng(0)
+    inp = rng.random((h, w, 4), dtype=np.float32)
+    di = dev.DImage.from_numpy(inp, lgx.RGBA32F)
+    out = dev.bilateral(di, g, radius=4)
+    out2 = dev.bilateral(out, g, radius=4)
+    torch.cuda.synchronize()
+    octn = orc.widen(gb["normal"])
+    ref = orc.bilateral(inp, gb["depth"], octn, 4)
+    ref2 = orc.bilateral(ref, gb["depth"], octn, 4)
+    assert orc.rel_l2(out.numpy(), ref) <= BILAT_TOL
+    assert orc.rel_l2(out2.numpy(), ref2) <= BILAT_TOL
+    # radius 0 passes through (denoiser.frag:82-86)
+    z = dev.bilateral(di, g, radius=0)
+    torch.cuda.synchronize()
+    assert z.numpy().tobytes() == inp.tobytes()
+
+
+def test_regression_denoise(dev):
+    sc, view, proj, gb = host_scene(4, 128, 96, "f32", 8, 2)
+    g = dev.GBufferDev.from_host(gb)
+    rng = np.random.default_rng(1)
+    inp = rng.random((96, 128, 4), dtype=np.float32)
+    di = dev.DImage.from_numpy(inp, lgx.RGBA32F)
+    do = dev.DImage(128, 96, lgx.RGBA32F)
+    i, gd, o = di.desc, g.desc, do.desc
+    lgx.check(lgx.lib().lgx_regression_denoise(C.byref(i), C.byref(gd), 2, C.byref(o), dev.stream_handle()))
+    torch.cuda.synchronize()
+    ref = orc.regression_denoise(inp, gb["depth"], 2)
+    assert do.numpy().tobytes() == ref.tobytes()
+
+
+def test_composite(dev):
+    sc, view, proj, gb = host_scene(5, 96, 64, "mixed", 8, 2)
+    g = dev.GBufferDev.from_host(gb)
+    rng = np.random.default_rng(2)
+    ind = rng.random((64, 96, 4), dtype=np.float32)
+    di = dev.DImage.from_numpy(ind, lgx.RGBA32F)
+    do = dev.DImage(96, 64, lgx.RGBA32F)
+    gd, i, o = g.desc, di.desc, do.desc
+    lgx.check(lgx.lib().lgx_composite(C.byref(gd), C.byref(i), C.byref(o), dev.stream_handle()))
+    torch.cuda.synchronize()
+    ref = orc.composite(orc.widen(gb["radiance"]), orc.widen(gb["albedo"]), ind)
+    assert np.allclose(do.numpy(), ref, rtol=1e-6, atol=1e-7)
+
+
+def test_full_plain_path_c1(dev, c1):
+    sc, view, proj, gb = c1[0]
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, 5, lgx.SAMPLER_HIZ_POINT).build()
+    o = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT)(view, proj, g, pyr)
+    for _ in range(2):
+        o = dev.bilateral(o, g, radius=4)
+    torch.cuda.synchronize()
+    ref = orc.plain_path(gb, view, proj, 16, 32, 5, lgx.SAMPLER_HIZ_POINT, passes=2, radius=4)
+    assert orc.rel_l2(o.numpy(), ref["out"]) <= GATHER_TOL
+
+
+def test_rejects_bad_arguments(dev):
+    L = lgx.lib()
+    a = dev.DImage(64, 48, lgx.RGBA32F)
+    b = dev.DImage(64, 40, lgx.RGBA32F)
+    ai, bi = a.desc, b.desc
+    assert L.lgx_deinterleave(C.byref(ai), C.byref(bi), 4, 4, None) == lgx.ESIZE
+    c = dev.DImage(66, 48, lgx.RGBA32F)
+    ci = c.desc
+    assert L.lgx_deinterleave(C.byref(ci), C.byref(ci), 4, 4, None) == lgx.ESIZE  # 4 does not divide 66
+    assert L.lgx_deinterleave(None, C.byref(ai), 4, 4, None) == lgx.EINVAL
+    lv = (lgx.Image * 2)(a.desc, b.desc)
+    assert L.lgx_mip_build(lv, 2, lgx.MIP_AVG, None) == lgx.ESIZE
+    lv = (lgx.Image * 2)(a.desc, dev.DImage(32, 24, lgx.RGBA32F).desc)
+    assert L.lgx_mip_build(lv, 2, lgx.MIP_MAX, None) == lgx.EFORMAT  # max-Z needs R32F
+    p = lgx.GatherParams()
+    p.width, p.height, p.dirs, p.grid = 64, 48, 0, 4
+    assert L.lgx_gather_workspace_bytes(C.byref(p)) == 0

@@ -55,50 +55,33 @@ def main():
     sampler = lgx.SAMPLER_HIZ_POINT
     scene = lgx.make_scene(SEED, 32, 8)
     view, proj = lgx.scene_matrices(scene, W, H)
-    stream = torch.cuda.current_stream()
     g = dev.GBufferDev.alloc(W, H, "mixed").render(scene)
-    pyr = dev.PyramidDev(g, LEVELS, sampler)
-    gather = dev.Gather(W, H, DIRS, STEPS, sampler, out_layout=lgx.OUT_SLICES)
-    sliced = dev.DImage(W, H, lgx.RGBA32F)
-    irr = dev.DImage(W, H, lgx.RGBA32F)
-    tmp = dev.DImage(W, H, lgx.RGBA32F)
-    ev_pairs = []
-
-    def frame(record=False):
-        pyr.build()
-        if record:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-        gather(view, proj, g, pyr, out=sliced)
-        if record:
-            e1.record(stream)
-            ev_pairs.append((e0, e1))
-        dev.permute(sliced, out=irr, inverse=True)
-        src, dst = irr, tmp
-        for _ in range(BILAT_PASSES):
-            dev.bilateral(src, g, out=dst, radius=BILAT_RADIUS)
-            src, dst = dst, src
-        return src
+    # The drop-in path: SsvgiRenderer passes declared on the C++ RenderGraph,
+    # timed per pass with hipEvents on the frame's stream.
+    renderer = dev.Renderer(W, H, DIRS, STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_BILATERAL,
+                            denoise_passes=BILAT_PASSES, denoise_radius=BILAT_RADIUS, kind="mixed", profile=True)
 
     # step counts (outside the timed region) for the algorithmic byte model
+    pyr = dev.PyramidDev(g, LEVELS, sampler).build()
     counts = torch.zeros((H, W), dtype=torch.int32, device="cuda")
-    pyr.build()
-    gather(view, proj, g, pyr, out=sliced, step_counts=counts)
+    dev.Gather(W, H, DIRS, STEPS, sampler)(view, proj, g, pyr, step_counts=counts)
     torch.cuda.synchronize()
     total_steps = int(counts.sum().item())
+    del pyr, counts
     gather_bytes = total_steps * (BD + BR) + W * H * (BC + BO)
 
     for _ in range(args.warmup):
-        frame()
+        renderer.render(g, view, proj)
     torch.cuda.synchronize()
+    renderer.pass_times(mode=2)  # reset the per-pass accumulation
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        out = frame(record=True)
+        renderer.render(g, view, proj)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    gather_ms = sum(a.elapsed_time(b) for a, b in ev_pairs) / len(ev_pairs)
+    passes = renderer.pass_times(mode=1)
+    gather_ms = sum(p["ms"] for p in passes if p["name"] == "IndirectLightPass")
     achieved = gather_bytes / (gather_ms * 1e-3) / 1e9
 
     result = {
@@ -119,27 +102,29 @@ def main():
                    "width": W, "height": H, "dirs": DIRS, "max_steps": STEPS, "levels": LEVELS, "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "gather_kernel", "kernel_ms": round(gather_ms, 4),
+                     "kernel": "gather_kernel (IndirectLightPass)", "kernel_ms": round(gather_ms, 4),
                      "algorithmic_bytes": gather_bytes, "steps_executed": total_steps},
+        "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(passes)},
     }
 
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(scene, view, proj, sampler, irr, args.cpu_sample_px)
+        result["cpu_baseline"] = cpu_baseline(g, view, proj, sampler, renderer.intermediate("indirect"),
+                                              args.cpu_sample_px)
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(scene, view, proj, sampler, gpu_irr, sample_px):
-    """FP32 oracle on a bounded sample: full-frame pyramids + gather on a
-    centred crop scaled to the frame + full-frame bilateral passes."""
+def cpu_baseline(g, view, proj, sampler, gpu_gather, sample_px):
+    """FP32 oracle on a bounded sample, fed the GPU's own G-buffer bytes:
+    full-frame pyramids + the gather on a centred crop scaled to the frame +
+    full-frame bilateral passes."""
     import numpy as np
-    from legitengine_amd import lgx
     from oracle import binding as orc
-    gb = lgx.render_gbuffer_cpu(scene, W, H, "mixed")
     threads = orc.lib().orc_max_threads()
-    depth = gb["depth"]
-    octn = orc.widen(gb["normal"])
+    depth = g.depth.numpy()
+    octn = orc.widen(g.normal.numpy())
+    radiance = orc.widen(g.radiance.numpy())
     t0 = time.perf_counter()
-    dep_b, rad_b, _, _ = orc.build_pyramids(depth, orc.widen(gb["radiance"]), LEVELS, sampler, True)
+    dep_b, rad_b, _, _ = orc.build_pyramids(depth, radiance, LEVELS, sampler, True)
     t_pyr = time.perf_counter() - t0
     side = int(np.sqrt(sample_px))
     x0, y0 = (W - side) // 2, (H - side) // 2
@@ -152,10 +137,10 @@ def cpu_baseline(scene, view, proj, sampler, gpu_irr, sample_px):
         img = orc.bilateral(img, depth, octn, BILAT_RADIUS)
     t_bil = time.perf_counter() - t0
     ms = (t_pyr + t_gather + t_bil) * 1e3
-    gpu = gpu_irr.numpy()[y0:y0 + side, x0:x0 + side]
+    gpu = gpu_gather[y0:y0 + side, x0:x0 + side]
     return {"value": round(ms, 1), "unit": "ms/frame", "cores": int(threads), "kind": "port",
-            "sample": f"FP32 oracle: full-frame pyramids and {BILAT_PASSES} bilateral passes, gather on a "
-                      f"{side}x{side} centre crop scaled by {W * H / (side * side):.1f} to the frame",
+            "sample": f"FP32 oracle on the GPU's G-buffer: full-frame pyramids and {BILAT_PASSES} bilateral "
+                      f"passes, gather on a {side}x{side} centre crop scaled by {W * H / (side * side):.1f}",
             "gather_rel_l2_on_crop": orc.rel_l2(gpu, ref)}
 
 

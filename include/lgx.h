@@ -194,6 +194,55 @@ int lgx_scene_matrices(const lgx_scene *s, int32_t width, int32_t height, float 
 int lgx_scene_render_cpu(const lgx_scene *s, const lgx_gbuffer *g);
 int lgx_scene_render_gpu(const lgx_scene *s, const lgx_gbuffer *g, lgx_stream_t stream);
 
+/* ---- the SSVGI renderer on the C++ RenderGraph (include/lgx/ssvgi.h) ----
+ * Replaces SSVGIRenderer's GI passes (SSVGIRenderer.h:207-342): the mips and
+ * Hi-Z, per-mip pre-blur, IndirectLightPass, interleave, DenoiserPass and
+ * the final gathering, declared per frame through RenderGraph::AddPass and
+ * executed on the given stream.  Creation allocates every viewport resource
+ * (RecreateSwapchainResources, SSVGIRenderer.h:24-29); frames allocate
+ * nothing.  The G-buffer is caller memory, rebound each frame. */
+typedef struct lgx_renderer lgx_renderer;
+enum { LGX_DENOISER_NONE = 0, LGX_DENOISER_BILATERAL = 1, LGX_DENOISER_REGRESSION = 2 };
+typedef struct {
+  int32_t width, height;
+  int32_t dirs, max_steps, levels, grid, sampler;
+  int32_t denoiser, denoise_passes, denoise_radius;
+  float depth_sharpness, normal_sharpness;
+  int32_t normal_format, albedo_format, radiance_format;
+  int32_t tile;
+  int32_t composite; /* 1: add the final gathering pass */
+  int32_t profile;   /* 1: per-pass hipEvent timings */
+  int32_t validate;  /* 1: passes may only touch the views and buffers they declare */
+  int32_t reserved;
+} lgx_renderer_config;
+
+int lgx_renderer_create(const lgx_renderer_config *cfg, lgx_renderer **out);
+/* Declares and runs one frame. */
+int lgx_renderer_render(lgx_renderer *r, const lgx_gbuffer *g, const float view[16], const float proj[16],
+                        lgx_stream_t stream);
+/* Declares one frame and captures it as a hipGraph (G-buffer pointers and
+ * camera are baked in); lgx_renderer_replay relaunches it. */
+int lgx_renderer_capture(lgx_renderer *r, const lgx_gbuffer *g, const float view[16], const float proj[16],
+                         lgx_stream_t stream);
+int lgx_renderer_replay(lgx_renderer *r, lgx_stream_t stream);
+/* Descriptor of a named intermediate ("depth_pyramid", "radiance_pyramid",
+ * "blurred_radiance", "blurred_depth", "indirect_slices", "indirect",
+ * "denoised", "final") at mip level `mip`. */
+int lgx_renderer_intermediate(lgx_renderer *r, const char *name, int32_t mip, lgx_image *out);
+/* Per-pass GPU times (hipEvents on the frame's stream, needs profile=1) as
+ * JSON [{"name":..,"ms":..},..]; waits for the frames concerned.
+ * mode 0: the last frame; 1: mean per pass over the frames since the last
+ * reset; 2: reset the accumulation (buf receives ""). */
+int lgx_renderer_pass_times(lgx_renderer *r, int32_t mode, char *buf, size_t cap);
+const char *lgx_renderer_last_error(const lgx_renderer *r);
+void lgx_renderer_destroy(lgx_renderer *r);
+
+/* Pitch-aware device-to-device copy of equal-extent, equal-format images (async). */
+int lgx_copy_image(const lgx_image *src, lgx_image *dst, lgx_stream_t stream);
+
+/* Number of gather launch shapes (lgx_gather_params.tile indexes them). */
+int lgx_gather_tiles(void);
+
 const char *lgx_strerror(int code);
 int32_t lgx_version(void);
 

@@ -19,12 +19,15 @@ LIB = os.path.join(HERE, "liblgx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LGX_ARCH", "gfx950")
 
-COMMON = [
+BASE = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wall", "-Wno-unused-function", "-Wno-unused-parameter",
 ]
+# HIP sources: device code for gfx950, packed-FP32 selection off
+HIP_FLAGS = [*BASE, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# host-only C++ sources (render graph, renderer): HIP runtime API headers only
+CXX_FLAGS = [*BASE, "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include"]
 
 
 def sources():
@@ -37,9 +40,9 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, "-c", src, "-o", obj]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", *HIP_FLAGS, "-c", src, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC, *COMMON, "-x", "c++", "-c", src, "-o", obj]
+        cmd = [HIPCC, *CXX_FLAGS, "-x", "c++", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

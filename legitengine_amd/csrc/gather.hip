@@ -35,8 +35,10 @@ template <class T> struct GatherArgs {
   T e0[3], ex[3], ey[3];  // unnormalised ray through screen point s: e0 + s.x*ex + s.y*ey
   T W, H, near_step, log_ratio, ambient;
   int w, h, grid, cells_x, cells_y, D, S_tab;
+  T rW, rH;        // RN(1/W), RN(1/H) for the exact division by the extent
   const T *dirs;   // [grid^2][D][2]
   const T *steps;  // [grid^2][S_tab][2] = (offset, lod)
+  const int *level_start;  // [grid^2][LGX_MAX_LEVELS + 1]: first step at or above each Hi-Z level
   const float *depth;
   int depth_pitch;
   const void *normal;
@@ -58,9 +60,27 @@ template <> LGX_D double fast_rsqrt<double>(double v) { return 1.0 / sqrt(v); }
 
 template <class T> LGX_D T fma_(T a, T b, T c) { return fma(a, b, c); }
 
-// Monotone stand-in for atan2(y, x) on (-pi, pi]: values in [-2, 2].
+template <class T> LGX_D T fast_rcp(T v);
+template <> LGX_D float fast_rcp<float>(float v) { return __builtin_amdgcn_rcpf(v); }
+template <> LGX_D double fast_rcp<double>(double v) { return 1.0 / v; }
+
+// q / b for the viewport extent b, correctly rounded: y = RN(q * RN(1/b)),
+// one fused correction with the exact residual q - y*b (Markstein).  The
+// result equals the IEEE quotient for every q that can select a texel; this
+// is checked exhaustively for the supported extents (tests/csrc/check_const_div.c),
+// so texel selection and the on-screen test match the shader's q / viewport.
+template <class T> LGX_D T div_extent(T q, T b, T rb) {
+  if (sizeof(T) == 8) return q / b;
+  const T y = q * rb;
+  const T e = fma(-y, b, q);
+  return fma(e, rb, y);
+}
+
+// Monotone stand-in for atan2(y, x) on (-pi, pi]: values in [-2, 2].  Only
+// its order matters; near-ties decide steps whose contribution h(theta,
+// theta_max) vanishes with the gap.
 template <class T> LGX_D T pseudo_angle(T y, T x) {
-  T r = x / (fabs(x) + fabs(y));
+  T r = x * fast_rcp(fabs(x) + fabs(y));
   return y >= T(0) ? T(1) - r : r - T(1);
 }
 
@@ -202,63 +222,124 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
     const T eu = dot3(eye, U), ev = dot3(eye, V);
     const T trc = rc * dot3(tangent, rayc);
     int j = 0;
-    for (; j < iters; j++) {
-      const T off = steps[2 * j], lod = steps[2 * j + 1];
-      const T qx = cx + dx * off, qy = cy + dy * off;  // :218
-      const T sqx = qx / a.W, sqy = qy / a.H;          // :219
-      if (!(sqx > T(0) && sqx < T(1) && sqy > T(0) && sqy < T(1))) break;
-      T r;
-      if (SAMPLER == LGX_SAMPLER_HIZ_POINT) {
-        const int l = point_level(lod, a.n_levels);
-        r = T(a.pd[l][point_index(sqx, sqy, a.lw[l], a.lh[l], a.pd_pitch[l])]);
-      } else {
-        T tmp[4];
-        sample_trilinear<T, LGX_R32F>((const void *const *)a.pd, a.pd_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod,
-                                      tmp, 1);
-        r = tmp[0];
-      }
-      // :241-252 sample delta projected on (eye, tangent)
-      const T len2 = fma_(off, fma_(off, vv, uv2), uu);
-      const T ri = r * fast_rsqrt(len2);
-      const T hyv = ri * fma_(off, tv, tu) - trc;
-      const T hxv = ri * fma_(off, ev, eu) - erc;
-      const T pa = pseudo_angle(hyv, hxv);
-      if (pa < pa_max) {  // :254
-        const T th = atan2(hyv, hxv);
-        const T qq = hxv * hxv + hyv * hyv;
-        const T c2 = (hxv * hxv - hyv * hyv) / qq, s2 = (T(2) * hxv * hyv) / qq;
-        T side = T(1);
-        {
-          const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
-          side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
-          side *= min(max(sqx * inv_width, T(0)), T(1));
-          side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
-          side *= min(max(sqy * inv_width, T(0)), T(1));
+    if (SAMPLER == LGX_SAMPLER_HIZ_POINT) {
+      // Level-major march: the Hi-Z level floor(lod) never decreases along a
+      // ray, so each level's constants stay in scalar registers for its run
+      // of steps.
+      const int *lstart = a.level_start + (size_t)k * (LGX_MAX_LEVELS + 1);
+      for (int l = 0; l < a.n_levels; l++) {
+        const int jend = (l == a.n_levels - 1) ? iters : min(iters, lstart[l + 1]);
+        if (j >= jend) continue;
+        const float *dl = a.pd[l];
+        const void *rl = a.pr[l];
+        const int dpitch = a.pd_pitch[l], rpitch = a.pr_pitch[l];
+        const T flw = T(a.lw[l]), flh = T(a.lh[l]), flw1 = T(a.lw[l] - 1), flh1 = T(a.lh[l] - 1);
+        for (; j < jend; j++) {
+          // every active lane is at the same step (all start a level at its
+          // first step and advance together), so the table read is uniform
+          const int ju = __builtin_amdgcn_readfirstlane(j);
+          const T off = steps[2 * ju];
+          const T qx = cx + dx * off, qy = cy + dy * off;  // :218
+          const T sqx = div_extent(qx, a.W, a.rW), sqy = div_extent(qy, a.H, a.rH);  // :219
+          if (!(sqx > T(0) && sqx < T(1) && sqy > T(0) && sqy < T(1))) goto dir_done;
+          const int ix = (int)min(max(floor(sqx * flw), T(0)), flw1);
+          const int iy = (int)min(max(floor(sqy * flh), T(0)), flh1);
+          const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
+          const T r = T(dl[ti]);
+          // :241-252 sample delta projected on (eye, tangent)
+          const T len2 = fma_(off, fma_(off, vv, uv2), uu);
+          const T ri = r * fast_rsqrt(len2);
+          const T hyv = ri * fma_(off, tv, tu) - trc;
+          const T hxv = ri * fma_(off, ev, eu) - erc;
+          const T pa = pseudo_angle(hyv, hxv);
+          if (pa < pa_max) {  // :254
+            const T th = atan2(hyv, hxv);
+            const T iq = fast_rcp(hxv * hxv + hyv * hyv);
+            const T c2 = (hxv * hxv - hyv * hyv) * iq, s2 = (T(2) * hxv * hyv) * iq;
+            T side = T(1);
+            {
+              const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
+              side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
+              side *= min(max(sqx * inv_width, T(0)), T(1));
+              side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
+              side *= min(max(sqy * inv_width, T(0)), T(1));
+            }
+            const T hc =
+                (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
+            const float4 v = Texel<RFMT>::load(rl, (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix);  // :256
+            L0 += T(v.x) * hc;  // :261
+            L1 += T(v.y) * hc;
+            L2 += T(v.z) * hc;
+            L0 -= ambient * hc;  // :262
+            L1 -= ambient * hc;
+            L2 -= ambient * hc;
+            max_h = th;  // :263
+            pa_max = pa;
+            c2max = c2;
+            s2max = s2;
+          }
         }
-        const T hc = (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
-        T ls[4];
+      }
+    } else {
+      for (; j < iters; j++) {
+        const T off = steps[2 * j], lod = steps[2 * j + 1];
+        const T qx = cx + dx * off, qy = cy + dy * off;  // :218
+        const T sqx = qx / a.W, sqy = qy / a.H;          // :219
+        if (!(sqx > T(0) && sqx < T(1) && sqy > T(0) && sqy < T(1))) break;
+        T r;
         if (SAMPLER == LGX_SAMPLER_HIZ_POINT) {
           const int l = point_level(lod, a.n_levels);
-          float4 v = Texel<RFMT>::load(a.pr[l], point_index(sqx, sqy, a.lw[l], a.lh[l], a.pr_pitch[l]));
-          ls[0] = T(v.x);
-          ls[1] = T(v.y);
-          ls[2] = T(v.z);
+          r = T(a.pd[l][point_index(sqx, sqy, a.lw[l], a.lh[l], a.pd_pitch[l])]);
         } else {
-          sample_trilinear<T, RFMT>((const void *const *)a.pr, a.pr_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod, ls,
-                                    3);
+          T tmp[4];
+          sample_trilinear<T, LGX_R32F>((const void *const *)a.pd, a.pd_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod,
+                                        tmp, 1);
+          r = tmp[0];
         }
-        L0 += ls[0] * hc;  // :261
-        L1 += ls[1] * hc;
-        L2 += ls[2] * hc;
-        L0 -= ambient * hc;  // :262
-        L1 -= ambient * hc;
-        L2 -= ambient * hc;
-        max_h = th;  // :263
-        pa_max = pa;
-        c2max = c2;
-        s2max = s2;
+        // :241-252 sample delta projected on (eye, tangent)
+        const T len2 = fma_(off, fma_(off, vv, uv2), uu);
+        const T ri = r * fast_rsqrt(len2);
+        const T hyv = ri * fma_(off, tv, tu) - trc;
+        const T hxv = ri * fma_(off, ev, eu) - erc;
+        const T pa = pseudo_angle(hyv, hxv);
+        if (pa < pa_max) {  // :254
+          const T th = atan2(hyv, hxv);
+          const T qq = hxv * hxv + hyv * hyv;
+          const T c2 = (hxv * hxv - hyv * hyv) / qq, s2 = (T(2) * hxv * hyv) / qq;
+          T side = T(1);
+          {
+            const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
+            side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
+            side *= min(max(sqx * inv_width, T(0)), T(1));
+            side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
+            side *= min(max(sqy * inv_width, T(0)), T(1));
+          }
+          const T hc = (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
+          T ls[4];
+          if (SAMPLER == LGX_SAMPLER_HIZ_POINT) {
+            const int l = point_level(lod, a.n_levels);
+            float4 v = Texel<RFMT>::load(a.pr[l], point_index(sqx, sqy, a.lw[l], a.lh[l], a.pr_pitch[l]));
+            ls[0] = T(v.x);
+            ls[1] = T(v.y);
+            ls[2] = T(v.z);
+          } else {
+            sample_trilinear<T, RFMT>((const void *const *)a.pr, a.pr_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod, ls,
+                                      3);
+          }
+          L0 += ls[0] * hc;  // :261
+          L1 += ls[1] * hc;
+          L2 += ls[2] * hc;
+          L0 -= ambient * hc;  // :262
+          L1 -= ambient * hc;
+          L2 -= ambient * hc;
+          max_h = th;  // :263
+          pa_max = pa;
+          c2max = c2;
+          s2max = s2;
+        }
       }
     }
+  dir_done:
     if (COUNT) step_total += j;
     sum0 += T(2) * L0 / T(a.D);  // :268
     sum1 += T(2) * L1 / T(a.D);
@@ -329,15 +410,21 @@ static GatherTableShape table_shape(const lgx_gather_params *p) {
 }
 
 // Tables in the shader's own arithmetic and evaluation order
-// (indirectLighting.frag:161-177 directions, :217 offsets, :234 lod).
-template <class T> static void fill_tables(const lgx_gather_params *p, const GatherTableShape &s, std::vector<T> &out) {
+// (indirectLighting.frag:161-177 directions, :217 offsets, :234 lod), then
+// per slice the first step of each Hi-Z level floor(lod) (clamped at 0).
+// Returns false if a slice's level sequence ever decreases (the level-major
+// march requires it non-decreasing; it is, as off and lod grow with j).
+template <class T>
+static bool fill_tables(const lgx_gather_params *p, const GatherTableShape &s, std::vector<unsigned char> &out) {
   const T pi = T(3.1415f);
   const T W = T(p->width);
   const T near_step = W / T(1000);
   const int D = p->dirs;
-  out.assign((size_t)s.g2 * D * 2 + (size_t)s.g2 * s.S_tab * 2, T(0));
-  T *dirs = out.data();
-  T *steps = dirs + (size_t)s.g2 * D * 2;
+  const size_t nd = (size_t)s.g2 * D * 2, ns = (size_t)s.g2 * s.S_tab * 2;
+  out.assign((nd + ns) * sizeof(T) + (size_t)s.g2 * (LGX_MAX_LEVELS + 1) * sizeof(int32_t), 0);
+  T *dirs = (T *)out.data();
+  T *steps = dirs + nd;
+  int32_t *lstart = (int32_t *)(steps + ns);
   for (int k = 0; k < s.g2; k++) {
     float angf, linf;
     hammersley(k, s.g2, angf, linf);
@@ -349,13 +436,20 @@ template <class T> static void fill_tables(const lgx_gather_params *p, const Gat
       dirs[((size_t)k * D + d) * 2 + 1] = std::sin(screen_ang);
     }
     const T ratio = T(2) * pi / T(D) + T(1);
+    int prev_level = 0;
+    for (int l = 0; l <= LGX_MAX_LEVELS; l++) lstart[k * (LGX_MAX_LEVELS + 1) + l] = l == 0 ? 0 : s.S_tab;
     for (int j = 0; j < s.S_tab; j++) {
       T off = near_step * std::pow(ratio, T(j) + lin_offset) + T(1) - near_step;
       T lod = std::log(std::max(T(0), T(2) * pi / T(D) * (off - T(1)) * T(0.5f))) / std::log(T(2)) + T(-2);
       steps[((size_t)k * s.S_tab + j) * 2 + 0] = off;
       steps[((size_t)k * s.S_tab + j) * 2 + 1] = lod;
+      int level = lod > T(0) ? (int)std::min(std::floor(lod), T(LGX_MAX_LEVELS)) : 0;
+      if (level < prev_level) return false;
+      for (int l = prev_level + 1; l <= level; l++) lstart[k * (LGX_MAX_LEVELS + 1) + l] = j;
+      prev_level = std::max(prev_level, level);
     }
   }
+  return true;
 }
 
 template <class T> static void frame_constants(const lgx_gather_params *p, GatherArgs<T> &a) {
@@ -430,24 +524,18 @@ using namespace lgx;
 extern "C" size_t lgx_gather_workspace_bytes(const lgx_gather_params *p) {
   if (!gather_params_ok(p)) return 0;
   GatherTableShape s = table_shape(p);
-  return ((size_t)s.g2 * s.D * 2 + (size_t)s.g2 * s.S_tab * 2) * s.elem;
+  return ((size_t)s.g2 * s.D * 2 + (size_t)s.g2 * s.S_tab * 2) * s.elem +
+         (size_t)s.g2 * (LGX_MAX_LEVELS + 1) * sizeof(int32_t);
 }
 
 extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
   if (!gather_params_ok(p) || !workspace) return LGX_EINVAL;
   GatherTableShape s = table_shape(p);
   if (s.S_tab > LGX_MAX_STEPS) return LGX_EUNSUPPORTED;
-  hipError_t e;
-  if (p->precision == LGX_PREC_F64) {
-    std::vector<double> t;
-    fill_tables<double>(p, s, t);
-    e = hipMemcpy(workspace, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
-  } else {
-    std::vector<float> t;
-    fill_tables<float>(p, s, t);
-    e = hipMemcpy(workspace, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
-  }
-  return hip_status(e);
+  std::vector<unsigned char> t;
+  bool ok = p->precision == LGX_PREC_F64 ? fill_tables<double>(p, s, t) : fill_tables<float>(p, s, t);
+  if (!ok) return LGX_EUNSUPPORTED;
+  return hip_status(hipMemcpy(workspace, t.data(), t.size(), hipMemcpyHostToDevice));
 }
 
 extern "C" int lgx_gather_tiles(void) { return kNumTiles; }
@@ -474,6 +562,9 @@ static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbu
   a.S_tab = s.S_tab;
   a.dirs = (const T *)ws;
   a.steps = (const T *)ws + (size_t)s.g2 * s.D * 2;
+  a.level_start = (const int *)(a.steps + (size_t)s.g2 * s.S_tab * 2);
+  a.rW = T(1) / a.W;
+  a.rH = T(1) / a.H;
   a.depth = (const float *)g->depth.ptr;
   a.depth_pitch = g->depth.pitch;
   a.normal = g->normal.ptr;

@@ -443,6 +443,16 @@ extern "C" int lgx_scene_render_gpu(const lgx_scene *s, const lgx_gbuffer *g, lg
   return hip_status(hipGetLastError());
 }
 
+extern "C" int lgx_copy_image(const lgx_image *src, lgx_image *dst, lgx_stream_t stream) {
+  if (!image_ok(src) || !image_ok(dst)) return LGX_EINVAL;
+  if (src->format != dst->format) return LGX_EFORMAT;
+  if (src->w != dst->w || src->h != dst->h) return LGX_ESIZE;
+  const size_t eb = (size_t)format_bytes(src->format);
+  return hip_status(hipMemcpy2DAsync(dst->ptr, (size_t)dst->pitch * eb, src->ptr, (size_t)src->pitch * eb,
+                                     (size_t)src->w * eb, (size_t)src->h, hipMemcpyDeviceToDevice,
+                                     (hipStream_t)stream));
+}
+
 extern "C" const char *lgx_strerror(int code) {
   switch (code) {
     case LGX_OK: return "ok";

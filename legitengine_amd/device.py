@@ -175,3 +175,71 @@ def permute(inp, out=None, inverse=False, gx=4, gy=4, stream=None):
     fn = lgx.lib().lgx_interleave if inverse else lgx.lib().lgx_deinterleave
     lgx.check(fn(C.byref(i), C.byref(o), gx, gy, stream_handle(stream)), "lgx_(de)interleave")
     return out
+
+
+class Renderer:
+    """The SSVGI renderer on the C++ RenderGraph (lgx_renderer_*)."""
+
+    def __init__(self, w, h, dirs=32, max_steps=64, levels=6, sampler=lgx.SAMPLER_HIZ_POINT, grid=4,
+                 denoiser=lgx.DENOISER_BILATERAL, denoise_passes=2, denoise_radius=4, kind="mixed", tile=0,
+                 composite=False, profile=False, validate=False):
+        f = lgx.GBUF_FORMATS[kind]
+        c = lgx.RendererConfig()
+        c.width, c.height, c.dirs, c.max_steps, c.levels, c.grid, c.sampler = w, h, dirs, max_steps, levels, grid, sampler
+        c.denoiser, c.denoise_passes, c.denoise_radius = denoiser, denoise_passes, denoise_radius
+        c.depth_sharpness, c.normal_sharpness = 10.5, 20.0
+        c.normal_format, c.albedo_format, c.radiance_format = f["normal"], f["albedo"], f["radiance"]
+        c.tile, c.composite, c.profile, c.validate = tile, int(composite), int(profile), int(validate)
+        self.cfg = c
+        h_ = C.c_void_p()
+        lgx.check(lgx.lib().lgx_renderer_create(C.byref(c), C.byref(h_)), "lgx_renderer_create")
+        self.h = h_
+
+    def _err(self, code, what):
+        if code != lgx.OK:
+            raise lgx.LgxError(code, f"{what}: {lgx.lib().lgx_renderer_last_error(self.h).decode()}")
+
+    def render(self, gb, view, proj, stream=None):
+        g = gb.desc
+        v = (C.c_float * 16)(*[float(x) for x in view])
+        p = (C.c_float * 16)(*[float(x) for x in proj])
+        self._err(lgx.lib().lgx_renderer_render(self.h, C.byref(g), v, p, stream_handle(stream)), "render")
+
+    def capture(self, gb, view, proj, stream=None):
+        g = gb.desc
+        v = (C.c_float * 16)(*[float(x) for x in view])
+        p = (C.c_float * 16)(*[float(x) for x in proj])
+        self._err(lgx.lib().lgx_renderer_capture(self.h, C.byref(g), v, p, stream_handle(stream)), "capture")
+
+    def replay(self, stream=None):
+        self._err(lgx.lib().lgx_renderer_replay(self.h, stream_handle(stream)), "replay")
+
+    def intermediate(self, name, mip=0):
+        """Copy of a named intermediate as a numpy array (synchronises)."""
+        im = lgx.Image()
+        self._err(lgx.lib().lgx_renderer_intermediate(self.h, name.encode(), mip, C.byref(im)), "intermediate")
+        dst = DImage(im.w, im.h, im.format)
+        d = dst.desc
+        lgx.check(lgx.lib().lgx_copy_image(C.byref(im), C.byref(d), stream_handle()), "lgx_copy_image")
+        torch.cuda.synchronize()
+        return dst.numpy()
+
+    def output(self):
+        return self.intermediate("final" if self.cfg.composite else "denoised")
+
+    def pass_times(self, mode=0):
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self._err(lgx.lib().lgx_renderer_pass_times(self.h, mode, buf, len(buf)), "pass_times")
+        return json.loads(buf.value.decode()) if mode != 2 else None
+
+    def close(self):
+        if self.h:
+            lgx.lib().lgx_renderer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -56,6 +56,17 @@ class Scene(C.Structure):
                 ("cam_vert", C.c_float), ("fovy", C.c_float), ("znear", C.c_float), ("zfar", C.c_float)]
 
 
+class RendererConfig(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("dirs", C.c_int32), ("max_steps", C.c_int32),
+                ("levels", C.c_int32), ("grid", C.c_int32), ("sampler", C.c_int32), ("denoiser", C.c_int32),
+                ("denoise_passes", C.c_int32), ("denoise_radius", C.c_int32), ("depth_sharpness", C.c_float),
+                ("normal_sharpness", C.c_float), ("normal_format", C.c_int32), ("albedo_format", C.c_int32),
+                ("radiance_format", C.c_int32), ("tile", C.c_int32), ("composite", C.c_int32),
+                ("profile", C.c_int32), ("validate", C.c_int32), ("reserved", C.c_int32)]
+
+
+DENOISER_NONE, DENOISER_BILATERAL, DENOISER_REGRESSION = 0, 1, 2
+
 # Every entry point declared in include/lgx.h, with its ctypes signature.
 SIGNATURES = {
     "lgx_gather_workspace_bytes": (C.c_size_t, [C.POINTER(GatherParams)]),
@@ -77,6 +88,17 @@ SIGNATURES = {
                                      C.POINTER(C.c_float)]),
     "lgx_scene_render_cpu": (C.c_int, [C.POINTER(Scene), C.POINTER(GBuffer)]),
     "lgx_scene_render_gpu": (C.c_int, [C.POINTER(Scene), C.POINTER(GBuffer), C.c_void_p]),
+    "lgx_renderer_create": (C.c_int, [C.POINTER(RendererConfig), C.POINTER(C.c_void_p)]),
+    "lgx_renderer_render": (C.c_int, [C.c_void_p, C.POINTER(GBuffer), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                      C.c_void_p]),
+    "lgx_renderer_capture": (C.c_int, [C.c_void_p, C.POINTER(GBuffer), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                       C.c_void_p]),
+    "lgx_renderer_replay": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lgx_renderer_intermediate": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(Image)]),
+    "lgx_renderer_pass_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_size_t]),
+    "lgx_renderer_last_error": (C.c_char_p, [C.c_void_p]),
+    "lgx_renderer_destroy": (None, [C.c_void_p]),
+    "lgx_copy_image": (C.c_int, [C.POINTER(Image), C.POINTER(Image), C.c_void_p]),
     "lgx_strerror": (C.c_char_p, [C.c_int]),
     "lgx_version": (C.c_int32, []),
 }

@@ -94,7 +94,9 @@ typedef struct {
 
 enum { LGX_MIP_AVG = 0, LGX_MIP_MAX = 1 };                         /* MipBuilder::FilterTypes (MipBuilder.h:136-140) */
 enum { LGX_SAMPLER_TRILINEAR = 0, LGX_SAMPLER_HIZ_POINT = 1 };    /* depth/radiance fetch mode */
-enum { LGX_PREC_F32 = 0, LGX_PREC_F64 = 1 };
+/* F32: float math and sums; F64: double math and sums; F32_ACC64: float
+ * per-step math, double accumulation of the contributions (output RGBA64F). */
+enum { LGX_PREC_F32 = 0, LGX_PREC_F64 = 1, LGX_PREC_F32_ACC64 = 2 };
 enum { LGX_OUT_INTERLEAVED = 0, LGX_OUT_SLICES = 1 };
 
 typedef struct {
@@ -108,8 +110,8 @@ typedef struct {
   int32_t precision; /* LGX_PREC_*: F64 computes and accumulates in double (output RGBA64F) */
   int32_t out_layout;/* LGX_OUT_*: SLICES writes the output deinterleaved (grid x grid slices) */
   int32_t *step_counts; /* optional device int32[W*H]: on-screen steps per pixel (sum over dirs), NULL = off */
-  int32_t tile;      /* 0 = default/autotuned launch shape, else an lgx_gather_tiles() index */
-  int32_t reserved[2];
+  int32_t tile;      /* launch shape: an index below lgx_gather_tiles() (0 = default) */
+  int32_t roi[4];    /* x0, y0, w, h: gather only these pixels (w = 0: the whole frame); output stays full-frame */
 } lgx_gather_params;
 
 /* Workspace for the gather: per-(pattern, direction) screen directions and
@@ -184,6 +186,9 @@ typedef struct {
 
 /* Host only: scene from a seed (SplitMix64), boxes <= 32, spheres <= 8. */
 int lgx_scene_make(uint64_t seed, int32_t n_boxes, int32_t n_spheres, lgx_scene *out);
+/* Host only: re-samples the camera (free position, aim) from pose_seed,
+ * keeping the room, boxes, spheres and light. */
+int lgx_scene_pose(lgx_scene *s, uint64_t pose_seed);
 /* Host only: view and projection matrices for the scene camera:
  * view = inverse(translate(pos) * rotY(hor) * rotX(vert)),
  * proj = perspectiveRH_NO(fovy, w/h, znear, zfar) * scale(1,-1,-1)
@@ -236,6 +241,43 @@ int lgx_renderer_intermediate(lgx_renderer *r, const char *name, int32_t mip, lg
 int lgx_renderer_pass_times(lgx_renderer *r, int32_t mode, char *buf, size_t cap);
 const char *lgx_renderer_last_error(const lgx_renderer *r);
 void lgx_renderer_destroy(lgx_renderer *r);
+
+/* ---- learned denoiser: BF16 U-Net on the matrix cores ----
+ * Replaces the regression DenoiserPass (SSVGIRenderer.h:265-302,
+ * denoiser.frag:112-184).  Three levels, 3x3 convolutions with 32/64/128
+ * channels, ReLU, average-pool down / nearest-upsample up with concatenated
+ * skips, 1x1 head predicting a correction of the noisy irradiance.  BF16
+ * activations and weights, FP32 accumulation, FP32 master weights + Adam.
+ * Input record: 12 FP32 per pixel = noisy irradiance rgb, radial depth,
+ * octahedral normal xy, albedo rgb, 3 unused.  Images are [n][h][w] with h
+ * and w multiples of 4.  Create allocates every buffer for the largest
+ * n x h x w; the other calls allocate nothing. */
+typedef struct lgx_unet lgx_unet;
+int64_t lgx_unet_param_count(void);
+int lgx_unet_create(int32_t max_n, int32_t max_h, int32_t max_w, uint64_t seed, lgx_unet **out);
+/* out: device RGBA32F [n][h][w] (alpha 1). */
+int lgx_unet_forward(lgx_unet *u, const float *x12, int32_t n, int32_t h, int32_t w, float *out, lgx_stream_t stream);
+/* Forward + L1-loss backward against target (device RGBA32F [n][h][w]);
+ * gradients kept inside; loss_out (device float, optional) = mean |out - target| over rgb. */
+int lgx_unet_backward(lgx_unet *u, const float *x12, const float *target, int32_t n, int32_t h, int32_t w,
+                      float *loss_out, lgx_stream_t stream);
+int lgx_unet_adam_step(lgx_unet *u, float lr, lgx_stream_t stream);
+int lgx_unet_train_step(lgx_unet *u, const float *x12, const float *target, int32_t n, int32_t h, int32_t w, float lr,
+                        float *loss_out, lgx_stream_t stream);
+/* Device pointer of the last forward's output inside the network (RGBA32F). */
+int lgx_unet_output(lgx_unet *u, float **out);
+/* Host copies of the state, synchronous. which: 0 parameters, 1 gradients
+ * (get only), 2 Adam first moment, 3 Adam second moment; flat FP32 in the
+ * layout [conv l: W[co][ci][3][3], b[co]] x 10, head W[3][32], b[3]. */
+int lgx_unet_get_state(lgx_unet *u, int32_t which, float *host, int64_t n, int32_t *adam_t);
+int lgx_unet_set_state(lgx_unet *u, int32_t which, const float *host, int64_t n, int32_t adam_t);
+/* Inference from a G-buffer and the noisy irradiance (RGBA32F); out RGBA32F, unpitched. */
+int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_image *noisy, lgx_image *out, lgx_stream_t stream);
+/* One wave computes D(16x16) = A(16x32, row-major) * B(32x16, row-major) on the
+ * matrix core through the operand layout the convolutions use (device FP32
+ * pointers; operands rounded to BF16). */
+int lgx_unet_mfma_probe(const float *A, const float *B, float *D, lgx_stream_t stream);
+void lgx_unet_destroy(lgx_unet *u);
 
 /* Pitch-aware device-to-device copy of equal-extent, equal-format images (async). */
 int lgx_copy_image(const lgx_image *src, lgx_image *dst, lgx_stream_t stream);

@@ -52,6 +52,8 @@ template <class T> struct GatherArgs {
   int out_pitch, out_slices;
   int *step_counts;
   int tile_w, tile_h, tiles_x, n_blocks;
+  int sx0, sy0, lcells_x, lcells_y;  // slice cells covered by the launch (region of interest)
+  int rx0, ry0, rx1, ry1;            // pixels gathered: [rx0, rx1) x [ry0, ry1)
 };
 
 template <class T> LGX_D T fast_rsqrt(T v);
@@ -152,17 +154,21 @@ template <class T> LGX_D size_t point_index(T sx, T sy, int lw, int lh, int pitc
   return (size_t)j * pitch + i;
 }
 
-template <class T, int SAMPLER, int RFMT, int NFMT, bool COUNT>
+// T: per-step arithmetic; A: accumulation of the slice contributions and of
+// the direction sum (float/float, double/double, or float/double).
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT>
 __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
   const uint32_t b = xcd_remap(blockIdx.x, (uint32_t)a.n_blocks);
   const int g2 = a.grid * a.grid;
   const int k = (int)(b % (uint32_t)g2);
   const int t = (int)(b / (uint32_t)g2);
   const int kx = k % a.grid, ky = k / a.grid;
-  const int sx = (t % a.tiles_x) * a.tile_w + (int)(threadIdx.x % (uint32_t)a.tile_w);
-  const int sy = (t / a.tiles_x) * a.tile_h + (int)(threadIdx.x / (uint32_t)a.tile_w);
+  const int lx = (t % a.tiles_x) * a.tile_w + (int)(threadIdx.x % (uint32_t)a.tile_w);
+  const int ly = (t / a.tiles_x) * a.tile_h + (int)(threadIdx.x / (uint32_t)a.tile_w);
+  if (lx >= a.lcells_x || ly >= a.lcells_y) return;
+  const int sx = a.sx0 + lx, sy = a.sy0 + ly;
   const int x = sx * a.grid + kx, y = sy * a.grid + ky;
-  if (sx >= a.cells_x || sy >= a.cells_y || x >= a.w || y >= a.h) return;
+  if (x < a.rx0 || x >= a.rx1 || y < a.ry0 || y >= a.ry1) return;
 
   const Vec3<T> cam{a.cam[0], a.cam[1], a.cam[2]};
   const T cx = T(x) + T(0.5), cy = T(y) + T(0.5);  // gl_FragCoord (:116)
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
 
   const T *dirs = a.dirs + (size_t)k * a.D * 2;
   const T *steps = a.steps + (size_t)k * a.S_tab * 2;
-  T sum0 = 0, sum1 = 0, sum2 = 0;
+  A sum0 = 0, sum1 = 0, sum2 = 0;
   int step_total = 0;
 
   for (int d = 0; d < a.D; d++) {
@@ -211,7 +217,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
     }
     // :209 ambient term: h(0, max_h)
     T h0 = T(0.25) * en * (T(1) - c2max) + T(0.25) * tn * (T(2) * max_h - s2max);
-    T L0 = ambient * h0, L1 = L0, L2 = L0;
+    A L0 = A(ambient * h0), L1 = L0, L2 = L0;
     // :212 step count, capped by the table (S)
     int iters = (int)(log(path / a.near_step) / a.log_ratio) + 1;
     iters = min(iters, a.S_tab);
@@ -267,12 +273,12 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
             const T hc =
                 (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
             const float4 v = Texel<RFMT>::load(rl, (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix);  // :256
-            L0 += T(v.x) * hc;  // :261
-            L1 += T(v.y) * hc;
-            L2 += T(v.z) * hc;
-            L0 -= ambient * hc;  // :262
-            L1 -= ambient * hc;
-            L2 -= ambient * hc;
+            L0 += A(T(v.x) * hc);  // :261
+            L1 += A(T(v.y) * hc);
+            L2 += A(T(v.z) * hc);
+            L0 -= A(ambient * hc);  // :262
+            L1 -= A(ambient * hc);
+            L2 -= A(ambient * hc);
             max_h = th;  // :263
             pa_max = pa;
             c2max = c2;
@@ -326,12 +332,12 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
             sample_trilinear<T, RFMT>((const void *const *)a.pr, a.pr_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod, ls,
                                       3);
           }
-          L0 += ls[0] * hc;  // :261
-          L1 += ls[1] * hc;
-          L2 += ls[2] * hc;
-          L0 -= ambient * hc;  // :262
-          L1 -= ambient * hc;
-          L2 -= ambient * hc;
+          L0 += A(ls[0] * hc);  // :261
+          L1 += A(ls[1] * hc);
+          L2 += A(ls[2] * hc);
+          L0 -= A(ambient * hc);  // :262
+          L1 -= A(ambient * hc);
+          L2 -= A(ambient * hc);
           max_h = th;  // :263
           pa_max = pa;
           c2max = c2;
@@ -341,9 +347,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
     }
   dir_done:
     if (COUNT) step_total += j;
-    sum0 += T(2) * L0 / T(a.D);  // :268
-    sum1 += T(2) * L1 / T(a.D);
-    sum2 += T(2) * L2 / T(a.D);
+    sum0 += A(2) * L0 / A(a.D);  // :268
+    sum1 += A(2) * L1 / A(a.D);
+    sum2 += A(2) * L2 / A(a.D);
   }
 
   int ox = x, oy = y;
@@ -352,7 +358,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
     oy = ky * a.cells_y + sy;
   }
   const size_t o = (size_t)oy * a.out_pitch + ox;
-  if (sizeof(T) == 4) {
+  if (sizeof(A) == 4) {
     ((float4 *)a.out)[o] = make_float4((float)sum0, (float)sum1, (float)sum2, 1.0f);  // :270
   } else {
     double *po = (double *)a.out + o * 4;
@@ -396,7 +402,7 @@ template <class T> static int table_steps(const lgx_gather_params *p) {
 static bool gather_params_ok(const lgx_gather_params *p) {
   return p && p->width > 0 && p->height > 0 && p->dirs >= 1 && p->dirs <= LGX_MAX_DIRS && p->grid >= 1 &&
          p->grid <= 8 && (p->sampler == LGX_SAMPLER_TRILINEAR || p->sampler == LGX_SAMPLER_HIZ_POINT) &&
-         (p->precision == LGX_PREC_F32 || p->precision == LGX_PREC_F64) &&
+         (p->precision == LGX_PREC_F32 || p->precision == LGX_PREC_F64 || p->precision == LGX_PREC_F32_ACC64) &&
          (p->out_layout == LGX_OUT_INTERLEAVED || p->out_layout == LGX_OUT_SLICES);
 }
 
@@ -493,22 +499,22 @@ template <class T> static void frame_constants(const lgx_gather_params *p, Gathe
   }
 }
 
-template <class T, int SAMPLER, int RFMT, int NFMT>
+template <class T, class A, int SAMPLER, int RFMT, int NFMT>
 static hipError_t launch_gather(const GatherArgs<T> &a, bool count, hipStream_t s) {
   dim3 grid((unsigned)a.n_blocks), block(256);
   if (count)
-    hipLaunchKernelGGL((gather_kernel<T, SAMPLER, RFMT, NFMT, true>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, true>), grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL((gather_kernel<T, SAMPLER, RFMT, NFMT, false>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
-template <class T, int SAMPLER>
+template <class T, class A, int SAMPLER>
 static hipError_t dispatch_fmt(const GatherArgs<T> &a, int rfmt, int nfmt, bool count, hipStream_t s) {
-  if (rfmt == LGX_RGBA16F && nfmt == LGX_RG16F) return launch_gather<T, SAMPLER, LGX_RGBA16F, LGX_RG16F>(a, count, s);
-  if (rfmt == LGX_RGBA32F && nfmt == LGX_RG32F) return launch_gather<T, SAMPLER, LGX_RGBA32F, LGX_RG32F>(a, count, s);
-  if (rfmt == LGX_RGBA16F && nfmt == LGX_RG32F) return launch_gather<T, SAMPLER, LGX_RGBA16F, LGX_RG32F>(a, count, s);
-  if (rfmt == LGX_RGBA32F && nfmt == LGX_RG16F) return launch_gather<T, SAMPLER, LGX_RGBA32F, LGX_RG16F>(a, count, s);
+  if (rfmt == LGX_RGBA16F && nfmt == LGX_RG16F) return launch_gather<T, A, SAMPLER, LGX_RGBA16F, LGX_RG16F>(a, count, s);
+  if (rfmt == LGX_RGBA32F && nfmt == LGX_RG32F) return launch_gather<T, A, SAMPLER, LGX_RGBA32F, LGX_RG32F>(a, count, s);
+  if (rfmt == LGX_RGBA16F && nfmt == LGX_RG32F) return launch_gather<T, A, SAMPLER, LGX_RGBA16F, LGX_RG32F>(a, count, s);
+  if (rfmt == LGX_RGBA32F && nfmt == LGX_RG16F) return launch_gather<T, A, SAMPLER, LGX_RGBA32F, LGX_RG16F>(a, count, s);
   return hipErrorInvalidValue;
 }
 
@@ -540,7 +546,7 @@ extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
 
 extern "C" int lgx_gather_tiles(void) { return kNumTiles; }
 
-template <class T>
+template <class T, class A>
 static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
                        lgx_image *out, hipStream_t stream) {
   GatherTableShape s = table_shape(p);
@@ -586,15 +592,29 @@ static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbu
   if (tile < 0 || tile >= kNumTiles) return LGX_EUNSUPPORTED;
   a.tile_w = kTiles[tile][0];
   a.tile_h = kTiles[tile][1];
-  a.tiles_x = (a.cells_x + a.tile_w - 1) / a.tile_w;
-  int tiles_y = (a.cells_y + a.tile_h - 1) / a.tile_h;
+  a.rx0 = 0;
+  a.ry0 = 0;
+  a.rx1 = p->width;
+  a.ry1 = p->height;
+  if (p->roi[2] > 0) {
+    a.rx0 = p->roi[0];
+    a.ry0 = p->roi[1];
+    a.rx1 = p->roi[0] + p->roi[2];
+    a.ry1 = p->roi[1] + p->roi[3];
+  }
+  a.sx0 = a.rx0 / p->grid;
+  a.sy0 = a.ry0 / p->grid;
+  a.lcells_x = (a.rx1 + p->grid - 1) / p->grid - a.sx0;
+  a.lcells_y = (a.ry1 + p->grid - 1) / p->grid - a.sy0;
+  a.tiles_x = (a.lcells_x + a.tile_w - 1) / a.tile_w;
+  int tiles_y = (a.lcells_y + a.tile_h - 1) / a.tile_h;
   a.n_blocks = a.tiles_x * tiles_y * s.g2;
   hipError_t e;
   const int rf = pyr->radiance[0].format, nf = g->normal.format;
   if (p->sampler == LGX_SAMPLER_HIZ_POINT)
-    e = dispatch_fmt<T, LGX_SAMPLER_HIZ_POINT>(a, rf, nf, p->step_counts != nullptr, stream);
+    e = dispatch_fmt<T, A, LGX_SAMPLER_HIZ_POINT>(a, rf, nf, p->step_counts != nullptr, stream);
   else
-    e = dispatch_fmt<T, LGX_SAMPLER_TRILINEAR>(a, rf, nf, p->step_counts != nullptr, stream);
+    e = dispatch_fmt<T, A, LGX_SAMPLER_TRILINEAR>(a, rf, nf, p->step_counts != nullptr, stream);
   return hip_status(e);
 }
 
@@ -608,8 +628,11 @@ extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, con
   if (g->depth.w != p->width || g->depth.h != p->height || g->normal.w != p->width || g->normal.h != p->height)
     return LGX_ESIZE;
   if (out->w != p->width || out->h != p->height) return LGX_ESIZE;
-  if (out->format != (p->precision == LGX_PREC_F64 ? LGX_RGBA64F : LGX_RGBA32F)) return LGX_EFORMAT;
+  if (out->format != (p->precision == LGX_PREC_F32 ? LGX_RGBA32F : LGX_RGBA64F)) return LGX_EFORMAT;
   if (p->out_layout == LGX_OUT_SLICES && (p->width % p->grid || p->height % p->grid)) return LGX_ESIZE;
+  if (p->roi[2] > 0 && (p->roi[0] < 0 || p->roi[1] < 0 || p->roi[3] <= 0 || p->roi[0] + p->roi[2] > p->width ||
+                        p->roi[1] + p->roi[3] > p->height))
+    return LGX_ESIZE;
   if (pyr->n_levels < 1 || pyr->n_levels > LGX_MAX_LEVELS) return LGX_EUNSUPPORTED;
   const int rf = pyr->radiance[0].format;
   if (rf != LGX_RGBA16F && rf != LGX_RGBA32F) return LGX_EFORMAT;
@@ -622,6 +645,8 @@ extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, con
     ew /= 2;
     eh /= 2;
   }
-  if (p->precision == LGX_PREC_F64) return gather_impl<double>(p, workspace, g, pyr, out, (hipStream_t)stream);
-  return gather_impl<float>(p, workspace, g, pyr, out, (hipStream_t)stream);
+  if (p->precision == LGX_PREC_F64) return gather_impl<double, double>(p, workspace, g, pyr, out, (hipStream_t)stream);
+  if (p->precision == LGX_PREC_F32_ACC64)
+    return gather_impl<float, double>(p, workspace, g, pyr, out, (hipStream_t)stream);
+  return gather_impl<float, float>(p, workspace, g, pyr, out, (hipStream_t)stream);
 }

@@ -310,42 +310,10 @@ static void setup_args(const lgx_scene *s, const lgx_gbuffer *g, SceneKernelArgs
                  g->albedo.pitch, g->radiance.pitch, g->normal.format, g->albedo.format, g->radiance.format};
 }
 
-}  // namespace lgx
-
-using namespace lgx;
-
-extern "C" int lgx_scene_make(uint64_t seed, int32_t n_boxes, int32_t n_spheres, lgx_scene *out) {
-  if (!out || n_boxes < 0 || n_boxes > 32 || n_spheres < 0 || n_spheres > 8) return LGX_EINVAL;
-  memset(out, 0, sizeof(*out));
-  SplitMix64 r{seed};
-  for (int i = 0; i < 6; i++)
-    for (int c = 0; c < 3; c++) out->wall_albedo[i][c] = r.uni(0.1f, 0.9f);
-  out->n_boxes = n_boxes;
-  out->n_spheres = n_spheres;
-  for (int b = 0; b < n_boxes; b++) {
-    for (int i = 0; i < 3; i++) {
-      float ext = r.uni(0.05f, 0.3f);
-      float mn = r.uni(0.0f, 1.0f - ext);
-      out->box_min[b][i] = mn;
-      out->box_max[b][i] = mn + ext;
-    }
-    for (int c = 0; c < 3; c++) out->box_albedo[b][c] = r.uni(0.1f, 0.9f);
-  }
-  for (int k = 0; k < n_spheres; k++) {
-    float rad = r.uni(0.05f, 0.15f);
-    out->sph_radius[k] = rad;
-    for (int i = 0; i < 3; i++) out->sph_center[k][i] = r.uni(rad, 1.0f - rad);
-    for (int c = 0; c < 3; c++) out->sph_albedo[k][c] = r.uni(0.1f, 0.9f);
-  }
-  {
-    float lx = r.uni(-0.8f, 0.8f), lz = r.uni(-0.8f, 0.8f), ly = 1.0f;
-    float il = 1.0f / sqrtf(lx * lx + ly * ly + lz * lz);
-    out->light_dir[0] = lx * il;
-    out->light_dir[1] = ly * il;
-    out->light_dir[2] = lz * il;
-    out->light_intensity = 3.0f;
-  }
-  // camera at a free point: outside every box and sphere by a margin
+// Camera at a free point (outside every box and sphere by a margin), aimed
+// at a point in the lower middle of the room.
+static void sample_camera(lgx_scene *out, SplitMix64 &r) {
+  const int n_boxes = out->n_boxes, n_spheres = out->n_spheres;
   float pos[3] = {0.5f, 0.5f, 0.5f};
   for (int tries = 0; tries < 4096; tries++) {
     float c[3] = {r.uni(0.08f, 0.92f), r.uni(0.08f, 0.92f), r.uni(0.08f, 0.92f)};
@@ -387,9 +355,54 @@ extern "C" int lgx_scene_make(uint64_t seed, int32_t n_boxes, int32_t n_spheres,
     out->cam_hor = atan2f(fx, fz);
     out->cam_vert = -asinf(fy / fl);
   }
+}
+
+}  // namespace lgx
+
+using namespace lgx;
+
+extern "C" int lgx_scene_make(uint64_t seed, int32_t n_boxes, int32_t n_spheres, lgx_scene *out) {
+  if (!out || n_boxes < 0 || n_boxes > 32 || n_spheres < 0 || n_spheres > 8) return LGX_EINVAL;
+  memset(out, 0, sizeof(*out));
+  SplitMix64 r{seed};
+  for (int i = 0; i < 6; i++)
+    for (int c = 0; c < 3; c++) out->wall_albedo[i][c] = r.uni(0.1f, 0.9f);
+  out->n_boxes = n_boxes;
+  out->n_spheres = n_spheres;
+  for (int b = 0; b < n_boxes; b++) {
+    for (int i = 0; i < 3; i++) {
+      float ext = r.uni(0.05f, 0.3f);
+      float mn = r.uni(0.0f, 1.0f - ext);
+      out->box_min[b][i] = mn;
+      out->box_max[b][i] = mn + ext;
+    }
+    for (int c = 0; c < 3; c++) out->box_albedo[b][c] = r.uni(0.1f, 0.9f);
+  }
+  for (int k = 0; k < n_spheres; k++) {
+    float rad = r.uni(0.05f, 0.15f);
+    out->sph_radius[k] = rad;
+    for (int i = 0; i < 3; i++) out->sph_center[k][i] = r.uni(rad, 1.0f - rad);
+    for (int c = 0; c < 3; c++) out->sph_albedo[k][c] = r.uni(0.1f, 0.9f);
+  }
+  {
+    float lx = r.uni(-0.8f, 0.8f), lz = r.uni(-0.8f, 0.8f), ly = 1.0f;
+    float il = 1.0f / sqrtf(lx * lx + ly * ly + lz * lz);
+    out->light_dir[0] = lx * il;
+    out->light_dir[1] = ly * il;
+    out->light_dir[2] = lz * il;
+    out->light_intensity = 3.0f;
+  }
+  sample_camera(out, r);
   out->fovy = 1.04719755f;  // 60 degrees
   out->znear = 0.01f;       // SSVGIRenderer.h:58
   out->zfar = 1000.0f;
+  return LGX_OK;
+}
+
+extern "C" int lgx_scene_pose(lgx_scene *s, uint64_t pose_seed) {
+  if (!s) return LGX_EINVAL;
+  SplitMix64 r{pose_seed ^ 0x5DEECE66Dull};
+  sample_camera(s, r);
   return LGX_OK;
 }
 

@@ -1,0 +1,62 @@
+// U-Net denoiser: kernel arguments and launchers (unet_kernels.hip), used by
+// the layer graph in unet.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "lgx.h"
+
+namespace lgx {
+namespace unet {
+
+typedef __bf16 bf16;
+
+struct ConvArgs {
+  const bf16 *a;  // input A, NHWC, ca channels (half resolution when a_up)
+  int ca, a_up;
+  const bf16 *b;  // input B, NHWC, cb channels (concatenated after A), or null
+  int cb;
+  int n, h, w;    // output extent
+  const bf16 *wt; // packed [9][(ca+cb)/32][cout][32]
+  const float *bias;
+  int cout, relu;
+  bf16 *out0;     // channels [0, split)
+  int split;
+  bf16 *out1;     // channels [split, cout)
+  const bf16 *mask0, *mask1;
+};
+
+struct WgradArgs {
+  const bf16 *a;
+  int ca, a_up;
+  const bf16 *b;
+  int cb;
+  const bf16 *dy;  // NHWC cout
+  int cout;
+  int n, h, w;
+  int rows;        // rows per workgroup
+  float *dw;       // [9][ca+cb][cout], accumulated
+};
+
+hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
+hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s);
+hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s);
+hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);
+hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12, hipStream_t s);
+hipError_t avgpool(const bf16 *x, int n, int h, int w, int c, bf16 *y, hipStream_t s);
+hipError_t pool_bwd_add(const bf16 *dx_in, const bf16 *dpool, const bf16 *act, int n, int h, int w, int c, bf16 *dx,
+                        hipStream_t s);
+hipError_t upsample_bwd(const bf16 *dfull, const bf16 *act, int n, int hh, int ww, int c, bf16 *dhalf, hipStream_t s);
+hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, float4 *out,
+                    hipStream_t s);
+hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
+                    float inv_count, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s);
+hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                float bc1, float bc2, hipStream_t s);
+hipError_t fill(float *p, size_t n, float v, hipStream_t s);
+hipError_t pack_weights(const float *w, int cout, int cin_log, int cin_pad, bf16 *wf, bf16 *wt, hipStream_t s);
+hipError_t unpack_wgrad(const float *dw, int cout, int cin_log, int cin_pad, float *g, hipStream_t s);
+hipError_t mfma_probe(const float *A, const float *B, float *D, hipStream_t s);
+
+}  // namespace unet
+}  // namespace lgx

@@ -1,0 +1,445 @@
+// U-Net denoiser: layer graph, training step and C ABI (lgx_unet_*).
+//
+// Three levels, 3x3 convolutions with 32 / 64 / 128 channels, ReLU, 2x2
+// average pooling down, nearest x2 upsampling up with concatenated skips,
+// and a 1x1 head that predicts a correction added to the noisy input:
+//
+//   x (9 ch, padded to 32) -> e1a -> e1 (32) -> pool -> e2a -> e2 (64) -> pool
+//   -> b1 -> b2 (128) -> [up(b2), e2] (192) -> d2a -> d2 (64)
+//   -> [up(d2), e1] (96) -> d1a -> d1 (32) -> head (1x1, 3) + noisy irradiance
+//
+// Activations and weights BF16, accumulation FP32 (MFMA), FP32 master
+// weights updated by Adam.  113,280 multiply-adds per pixel (the padded first
+// layer adds 6,624).  Input record per pixel: noisy irradiance rgb, radial
+// depth, octahedral normal xy, albedo rgb.
+#include <math.h>
+
+#include <memory>
+#include <vector>
+
+#include "lgx_internal.h"
+#include "unet.h"
+
+namespace lgx {
+namespace unet {
+
+struct LayerSpec {
+  int cin, cin_pad, cout;
+};
+static const LayerSpec kLayers[10] = {
+    {9, 32, 32},    {32, 32, 32},  // e1a, e1
+    {32, 32, 64},   {64, 64, 64},  // e2a, e2
+    {64, 64, 128},  {128, 128, 128},  // b1, b2
+    {192, 192, 64}, {64, 64, 64},  // d2a, d2
+    {96, 96, 32},   {32, 32, 32},  // d1a, d1
+};
+static const int kHeadIn = 32, kHeadOut = 3;
+
+static size_t layer_params(int l) { return (size_t)kLayers[l].cout * kLayers[l].cin * 9 + kLayers[l].cout; }
+static size_t total_params() {
+  size_t n = 0;
+  for (int l = 0; l < 10; l++) n += layer_params(l);
+  return n + kHeadIn * kHeadOut + kHeadOut;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+class Net {
+ public:
+  int max_n, max_h, max_w;
+  std::vector<std::unique_ptr<DevBuf>> owned;
+  float *params, *grads, *adam_m, *adam_v;
+  size_t np;
+  size_t w_off[10], b_off[10], head_w_off, head_b_off;
+  bf16 *wf[10], *wt[10];
+  float *dw[10];  // gradient layout [9][cin_pad][cout]
+  // activations
+  bf16 *x32, *e1a, *e1, *p1, *e2a, *e2, *p2, *b1, *b2, *d2a, *d2, *d1a, *d1;
+  float *out, *x12_scratch, *loss;
+  // gradients of activations
+  bf16 *g_d1, *g_d1a, *g_upd2, *g_e1c, *g_d2, *g_d2a, *g_upb2, *g_e2c, *g_b2, *g_b1, *g_p2, *g_e2, *g_e2a, *g_p1, *g_e1,
+      *g_e1a;
+  int adam_t = 0;
+
+  void *alloc(size_t bytes) {
+    std::unique_ptr<DevBuf> b(new DevBuf);
+    b->bytes = (bytes + 255) / 256 * 256;
+    if (hipMalloc(&b->p, b->bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(LGX_EINVAL, "lgx_unet: hipMalloc failed");
+    }
+    void *p = b->p;
+    owned.push_back(std::move(b));
+    return p;
+  }
+  struct Error {
+    int code;
+    const char *what;
+    Error(int c, const char *w) : code(c), what(w) {}
+  };
+
+  Net(int n, int h, int w) : max_n(n), max_h(h), max_w(w) {
+    np = total_params();
+    size_t off = 0;
+    for (int l = 0; l < 10; l++) {
+      w_off[l] = off;
+      off += (size_t)kLayers[l].cout * kLayers[l].cin * 9;
+      b_off[l] = off;
+      off += kLayers[l].cout;
+    }
+    head_w_off = off;
+    off += kHeadIn * kHeadOut;
+    head_b_off = off;
+    params = (float *)alloc(np * 4);
+    grads = (float *)alloc(np * 4);
+    adam_m = (float *)alloc(np * 4);
+    adam_v = (float *)alloc(np * 4);
+    for (int l = 0; l < 10; l++) {
+      const LayerSpec &L = kLayers[l];
+      wf[l] = (bf16 *)alloc((size_t)9 * L.cin_pad * L.cout * 2);
+      wt[l] = l == 0 ? nullptr : (bf16 *)alloc((size_t)9 * L.cin_pad * L.cout * 2);
+      dw[l] = (float *)alloc((size_t)9 * L.cin_pad * L.cout * 4);
+    }
+    const size_t P = (size_t)n * h * w, P2 = P / 4, P4 = P / 16;
+    auto A = [&](size_t pix, int c) { return (bf16 *)alloc(pix * c * 2); };
+    x32 = A(P, 32);
+    e1a = A(P, 32);
+    e1 = A(P, 32);
+    p1 = A(P2, 32);
+    e2a = A(P2, 64);
+    e2 = A(P2, 64);
+    p2 = A(P4, 64);
+    b1 = A(P4, 128);
+    b2 = A(P4, 128);
+    d2a = A(P2, 64);
+    d2 = A(P2, 64);
+    d1a = A(P, 32);
+    d1 = A(P, 32);
+    out = (float *)alloc(P * 16);
+    x12_scratch = (float *)alloc((size_t)h * w * 48);
+    loss = (float *)alloc(4);
+    g_d1 = A(P, 32);
+    g_d1a = A(P, 32);
+    g_upd2 = A(P, 64);
+    g_e1c = A(P, 32);
+    g_d2 = A(P2, 64);
+    g_d2a = A(P2, 64);
+    g_upb2 = A(P2, 128);
+    g_e2c = A(P2, 64);
+    g_b2 = A(P4, 128);
+    g_b1 = A(P4, 128);
+    g_p2 = A(P4, 64);
+    g_e2 = A(P2, 64);
+    g_e2a = A(P2, 64);
+    g_p1 = A(P2, 32);
+    g_e1 = A(P, 32);
+    g_e1a = A(P, 32);
+  }
+
+  void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error((int)e, what);
+    }
+  }
+
+  void pack(hipStream_t s) {
+    for (int l = 0; l < 10; l++)
+      check(pack_weights(params + w_off[l], kLayers[l].cout, kLayers[l].cin, kLayers[l].cin_pad, wf[l], wt[l], s),
+            "pack_weights");
+  }
+
+  void conv(int l, const bf16 *a, int ca, int a_up, const bf16 *b, int cb, int n, int h, int w, bf16 *out_,
+            hipStream_t s) {
+    ConvArgs c{};
+    c.a = a;
+    c.ca = ca;
+    c.a_up = a_up;
+    c.b = b;
+    c.cb = cb;
+    c.n = n;
+    c.h = h;
+    c.w = w;
+    c.wt = wf[l];
+    c.bias = params + b_off[l];
+    c.cout = kLayers[l].cout;
+    c.relu = 1;
+    c.out0 = out_;
+    c.split = c.cout;
+    check(conv3x3(c, s), "conv3x3");
+  }
+
+  // input gradient of layer l: dy (cout channels) -> [split | rest] with masks
+  void conv_dx(int l, const bf16 *dy, int n, int h, int w, bf16 *o0, int split, bf16 *o1, const bf16 *m0,
+               const bf16 *m1, hipStream_t s) {
+    ConvArgs c{};
+    c.a = dy;
+    c.ca = kLayers[l].cout;
+    c.n = n;
+    c.h = h;
+    c.w = w;
+    c.wt = wt[l];
+    c.cout = kLayers[l].cin_pad;
+    c.out0 = o0;
+    c.split = split;
+    c.out1 = o1;
+    c.mask0 = m0;
+    c.mask1 = m1;
+    check(conv3x3(c, s), "conv3x3 input gradient");
+  }
+
+  void wgrad(int l, const bf16 *a, int ca, int a_up, const bf16 *b, int cb, const bf16 *dy, int n, int h, int w,
+             hipStream_t s) {
+    WgradArgs g{};
+    g.a = a;
+    g.ca = ca;
+    g.a_up = a_up;
+    g.b = b;
+    g.cb = cb;
+    g.dy = dy;
+    g.cout = kLayers[l].cout;
+    g.n = n;
+    g.h = h;
+    g.w = w;
+    const int groups = (ca + cb) / 32;
+    long blocks_per_row = (long)((w + 63) / 64) * n * groups;
+    int rows = (int)((long)h * blocks_per_row / 1024);
+    g.rows = rows < 1 ? 1 : (rows > 16 ? 16 : rows);
+    g.dw = dw[l];
+    check(conv3x3_wgrad(g, s), "conv3x3_wgrad");
+    check(channel_sum(dy, kLayers[l].cout, (size_t)n * h * w, grads + b_off[l], s), "channel_sum");
+  }
+
+  void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s) {
+    const size_t P = (size_t)n * h * w;
+    const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
+    check(pack_input(x12, P, x32, s), "pack_input");
+    conv(0, x32, 32, 0, nullptr, 0, n, h, w, e1a, s);
+    conv(1, e1a, 32, 0, nullptr, 0, n, h, w, e1, s);
+    check(avgpool(e1, n, h, w, 32, p1, s), "avgpool");
+    conv(2, p1, 32, 0, nullptr, 0, n, h2, w2, e2a, s);
+    conv(3, e2a, 64, 0, nullptr, 0, n, h2, w2, e2, s);
+    check(avgpool(e2, n, h2, w2, 64, p2, s), "avgpool");
+    conv(4, p2, 64, 0, nullptr, 0, n, h4, w4, b1, s);
+    conv(5, b1, 128, 0, nullptr, 0, n, h4, w4, b2, s);
+    conv(6, b2, 128, 1, e2, 64, n, h2, w2, d2a, s);
+    conv(7, d2a, 64, 0, nullptr, 0, n, h2, w2, d2, s);
+    conv(8, d2, 64, 1, e1, 32, n, h, w, d1a, s);
+    conv(9, d1a, 32, 0, nullptr, 0, n, h, w, d1, s);
+    check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, (float4 *)out_, s), "head_fwd");
+  }
+
+  void backward(const float *x12, const float *target, int n, int h, int w, hipStream_t s) {
+    const size_t P = (size_t)n * h * w;
+    const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
+    forward(x12, n, h, w, out, s);
+    check(fill(grads, np, 0.f, s), "fill");
+    check(fill(loss, 1, 0.f, s), "fill");
+    for (int l = 0; l < 10; l++) check(fill(dw[l], (size_t)9 * kLayers[l].cin_pad * kLayers[l].cout, 0.f, s), "fill");
+    check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P), g_d1,
+                   grads + head_w_off, grads + head_b_off, loss, s),
+          "head_bwd");
+    // d1 = relu(conv9(d1a))
+    wgrad(9, d1a, 32, 0, nullptr, 0, g_d1, n, h, w, s);
+    conv_dx(9, g_d1, n, h, w, g_d1a, 32, nullptr, d1a, nullptr, s);
+    // d1a = relu(conv8([up(d2), e1]))
+    wgrad(8, d2, 64, 1, e1, 32, g_d1a, n, h, w, s);
+    conv_dx(8, g_d1a, n, h, w, g_upd2, 64, g_e1c, nullptr, nullptr, s);
+    check(upsample_bwd(g_upd2, d2, n, h2, w2, 64, g_d2, s), "upsample_bwd");
+    // d2 = relu(conv7(d2a))
+    wgrad(7, d2a, 64, 0, nullptr, 0, g_d2, n, h2, w2, s);
+    conv_dx(7, g_d2, n, h2, w2, g_d2a, 64, nullptr, d2a, nullptr, s);
+    // d2a = relu(conv6([up(b2), e2]))
+    wgrad(6, b2, 128, 1, e2, 64, g_d2a, n, h2, w2, s);
+    conv_dx(6, g_d2a, n, h2, w2, g_upb2, 128, g_e2c, nullptr, nullptr, s);
+    check(upsample_bwd(g_upb2, b2, n, h4, w4, 128, g_b2, s), "upsample_bwd");
+    // b2 = relu(conv5(b1)), b1 = relu(conv4(p2))
+    wgrad(5, b1, 128, 0, nullptr, 0, g_b2, n, h4, w4, s);
+    conv_dx(5, g_b2, n, h4, w4, g_b1, 128, nullptr, b1, nullptr, s);
+    wgrad(4, p2, 64, 0, nullptr, 0, g_b1, n, h4, w4, s);
+    conv_dx(4, g_b1, n, h4, w4, g_p2, 64, nullptr, nullptr, nullptr, s);
+    // e2 feeds the pool and the skip
+    check(pool_bwd_add(g_e2c, g_p2, e2, n, h2, w2, 64, g_e2, s), "pool_bwd_add");
+    wgrad(3, e2a, 64, 0, nullptr, 0, g_e2, n, h2, w2, s);
+    conv_dx(3, g_e2, n, h2, w2, g_e2a, 64, nullptr, e2a, nullptr, s);
+    wgrad(2, p1, 32, 0, nullptr, 0, g_e2a, n, h2, w2, s);
+    conv_dx(2, g_e2a, n, h2, w2, g_p1, 32, nullptr, nullptr, nullptr, s);
+    check(pool_bwd_add(g_e1c, g_p1, e1, n, h, w, 32, g_e1, s), "pool_bwd_add");
+    wgrad(1, e1a, 32, 0, nullptr, 0, g_e1, n, h, w, s);
+    conv_dx(1, g_e1, n, h, w, g_e1a, 32, nullptr, e1a, nullptr, s);
+    wgrad(0, x32, 32, 0, nullptr, 0, g_e1a, n, h, w, s);
+    for (int l = 0; l < 10; l++)
+      check(unpack_wgrad(dw[l], kLayers[l].cout, kLayers[l].cin, kLayers[l].cin_pad, grads + w_off[l], s),
+            "unpack_wgrad");
+  }
+
+  void adam_step(float lr, hipStream_t s) {
+    adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = 1.f - powf(b1, (float)adam_t), bc2 = 1.f - powf(b2, (float)adam_t);
+    check(adam(params, grads, adam_m, adam_v, np, lr, b1, b2, eps, bc1, bc2, s), "adam");
+    pack(s);
+  }
+};
+
+// He-normal initialisation from SplitMix64 + Box-Muller; the head starts near
+// zero so the untrained network passes the noisy input through.
+static void init_params(std::vector<float> &p, uint64_t seed, const size_t *w_off, const size_t *b_off,
+                        size_t head_w_off) {
+  uint64_t s = seed;
+  auto next = [&]() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  auto uni = [&]() { return ((double)(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+  auto normal = [&]() { return sqrt(-2.0 * log(uni())) * cos(6.283185307179586 * uni()); };
+  std::fill(p.begin(), p.end(), 0.f);
+  for (int l = 0; l < 10; l++) {
+    const double std_ = sqrt(2.0 / (kLayers[l].cin * 9));
+    const size_t nw = (size_t)kLayers[l].cout * kLayers[l].cin * 9;
+    for (size_t i = 0; i < nw; i++) p[w_off[l] + i] = (float)(normal() * std_);
+  }
+  for (int i = 0; i < kHeadIn * kHeadOut; i++) p[head_w_off + i] = (float)(normal() * 0.01);
+  (void)b_off;
+}
+
+}  // namespace unet
+}  // namespace lgx
+
+using lgx::unet::Net;
+
+struct lgx_unet {
+  std::unique_ptr<Net> net;
+};
+
+template <class F> static int unet_guard(F fn) {
+  try {
+    fn();
+    return LGX_OK;
+  } catch (const Net::Error &e) {
+    return e.code;
+  } catch (...) {
+    return LGX_EINVAL;
+  }
+}
+
+extern "C" int64_t lgx_unet_param_count(void) { return (int64_t)lgx::unet::total_params(); }
+
+extern "C" int lgx_unet_create(int32_t max_n, int32_t max_h, int32_t max_w, uint64_t seed, lgx_unet **out) {
+  if (!out || max_n < 1 || max_h < 4 || max_w < 4 || max_h % 4 || max_w % 4) return LGX_EINVAL;
+  *out = nullptr;
+  lgx_unet *u = new lgx_unet;
+  int rc = unet_guard([&]() {
+    u->net.reset(new Net(max_n, max_h, max_w));
+    Net &N = *u->net;
+    std::vector<float> p(N.np);
+    lgx::unet::init_params(p, seed, N.w_off, N.b_off, N.head_w_off);
+    N.check(hipMemcpy(N.params, p.data(), N.np * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    N.check(lgx::unet::fill(N.adam_m, N.np, 0.f, nullptr), "fill");
+    N.check(lgx::unet::fill(N.adam_v, N.np, 0.f, nullptr), "fill");
+    N.pack(nullptr);
+    N.check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  });
+  if (rc != LGX_OK) {
+    delete u;
+    return rc;
+  }
+  *out = u;
+  return LGX_OK;
+}
+
+static bool shape_ok(const lgx_unet *u, int32_t n, int32_t h, int32_t w) {
+  return u && u->net && n >= 1 && h >= 4 && w >= 4 && h % 4 == 0 && w % 4 == 0 &&
+         (int64_t)n * h * w <= (int64_t)u->net->max_n * u->net->max_h * u->net->max_w && h * w <= u->net->max_h * u->net->max_w;
+}
+
+extern "C" int lgx_unet_forward(lgx_unet *u, const float *x12, int32_t n, int32_t h, int32_t w, float *out,
+                                lgx_stream_t stream) {
+  if (!shape_ok(u, n, h, w) || !x12 || !out) return LGX_EINVAL;
+  return unet_guard([&]() { u->net->forward(x12, n, h, w, out, (hipStream_t)stream); });
+}
+
+extern "C" int lgx_unet_backward(lgx_unet *u, const float *x12, const float *target, int32_t n, int32_t h, int32_t w,
+                                 float *loss_out, lgx_stream_t stream) {
+  if (!shape_ok(u, n, h, w) || !x12 || !target) return LGX_EINVAL;
+  return unet_guard([&]() {
+    Net &N = *u->net;
+    N.backward(x12, target, n, h, w, (hipStream_t)stream);
+    if (loss_out)
+      N.check(hipMemcpyAsync(loss_out, N.loss, 4, hipMemcpyDeviceToDevice, (hipStream_t)stream), "hipMemcpyAsync");
+  });
+}
+
+extern "C" int lgx_unet_adam_step(lgx_unet *u, float lr, lgx_stream_t stream) {
+  if (!u || !u->net) return LGX_EINVAL;
+  return unet_guard([&]() { u->net->adam_step(lr, (hipStream_t)stream); });
+}
+
+extern "C" int lgx_unet_train_step(lgx_unet *u, const float *x12, const float *target, int32_t n, int32_t h, int32_t w,
+                                   float lr, float *loss_out, lgx_stream_t stream) {
+  int rc = lgx_unet_backward(u, x12, target, n, h, w, loss_out, stream);
+  if (rc != LGX_OK) return rc;
+  return lgx_unet_adam_step(u, lr, stream);
+}
+
+extern "C" int lgx_unet_output(lgx_unet *u, float **out) {
+  if (!u || !u->net || !out) return LGX_EINVAL;
+  *out = u->net->out;
+  return LGX_OK;
+}
+
+// which: 0 params, 1 grads, 2 adam m, 3 adam v (host copies, synchronous)
+extern "C" int lgx_unet_get_state(lgx_unet *u, int32_t which, float *host, int64_t n, int32_t *adam_t) {
+  if (!u || !u->net || !host || which < 0 || which > 3 || n != (int64_t)u->net->np) return LGX_EINVAL;
+  return unet_guard([&]() {
+    Net &N = *u->net;
+    float *src = which == 0 ? N.params : which == 1 ? N.grads : which == 2 ? N.adam_m : N.adam_v;
+    N.check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    N.check(hipMemcpy(host, src, N.np * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (adam_t) *adam_t = N.adam_t;
+  });
+}
+
+extern "C" int lgx_unet_set_state(lgx_unet *u, int32_t which, const float *host, int64_t n, int32_t adam_t) {
+  if (!u || !u->net || !host || which < 0 || which > 3 || which == 1 || n != (int64_t)u->net->np) return LGX_EINVAL;
+  return unet_guard([&]() {
+    Net &N = *u->net;
+    float *dst = which == 0 ? N.params : which == 2 ? N.adam_m : N.adam_v;
+    N.check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    N.check(hipMemcpy(dst, host, N.np * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    if (adam_t >= 0) N.adam_t = adam_t;
+    if (which == 0) N.pack(nullptr);
+    N.check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  });
+}
+
+extern "C" int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_image *noisy, lgx_image *out,
+                                lgx_stream_t stream) {
+  if (!u || !u->net || !g || !noisy || !out) return LGX_EINVAL;
+  if (noisy->format != LGX_RGBA32F || out->format != LGX_RGBA32F) return LGX_EFORMAT;
+  if (out->w != noisy->w || out->h != noisy->h || out->pitch != out->w || g->depth.w != noisy->w ||
+      g->depth.h != noisy->h)
+    return LGX_ESIZE;
+  if (!shape_ok(u, 1, noisy->h, noisy->w)) return LGX_ESIZE;
+  return unet_guard([&]() {
+    Net &N = *u->net;
+    hipStream_t s = (hipStream_t)stream;
+    N.check(lgx::unet::pack_gbuffer(*noisy, *g, N.x12_scratch, s), "pack_gbuffer");
+    N.forward(N.x12_scratch, 1, noisy->h, noisy->w, (float *)out->ptr, s);
+  });
+}
+
+extern "C" int lgx_unet_mfma_probe(const float *A, const float *B, float *D, lgx_stream_t stream) {
+  if (!A || !B || !D) return LGX_EINVAL;
+  return (int)lgx::unet::mfma_probe(A, B, D, (hipStream_t)stream);
+}
+
+extern "C" void lgx_unet_destroy(lgx_unet *u) { delete u; }

@@ -1,0 +1,574 @@
+// U-Net denoiser kernels for gfx950: BF16 activations and weights, FP32
+// accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// Replaces the reference's regression DenoiserPass (SSVGIRenderer.h:265-302,
+// denoiser.frag:112-184) with a learned filter; the layer graph lives in
+// unet.cpp.
+//
+// Layouts: activations NHWC bf16 with channel counts that are multiples of
+// 32 (convolution inputs) or 16 (outputs).  Convolution weights are packed
+// per K-chunk: [tap 9][Cin/32][Cout][32], so one lane's MFMA operand (8
+// consecutive input channels of one output channel) is a single 16-byte load.
+//
+// MFMA 16x16x32 operand layout (gfx950, wave64, lane l, item i of 8 / 4):
+//   A (16 x 32, row m, col k):  m = l % 16, k = 8 * (l / 16) + i
+//   B (32 x 16, row k, col n):  n = l % 16, k = 8 * (l / 16) + i
+//   D (16 x 16):                n = l % 16, m = 4 * (l / 16) + i
+// The convolution maps M = output channels, N = pixels, K = (tap, Cin):
+// A is read from the packed weights, B straight from the NHWC activations
+// (16 bytes = 8 channels of one pixel per lane), and each lane's result is 4
+// consecutive output channels of one pixel (one 8-byte store).
+#include <math.h>
+
+#include "lgx_internal.h"
+#include "unet.h"
+
+namespace lgx {
+namespace unet {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+LGX_D float bf2f(bf16 v) { return (float)v; }
+LGX_D bf16 f2bf(float v) { return (bf16)v; }
+
+// ------------------------------------------------------------ convolution
+
+// Input addressing shared by forward, input-gradient and weight-gradient
+// kernels: channels [0, ca) come from A (optionally nearest-upsampled from
+// half resolution), [ca, ca+cb) from B.
+struct ConvInput {
+  const bf16 *a;
+  int ca, a_up;
+  const bf16 *b;
+  int cb;
+};
+
+// 16 bytes (8 channels starting at c, c % 8 == 0, within one 32-chunk) of
+// pixel (n, y, x) of the logical input; zero outside the image.
+LGX_D bf16x8 load8(const ConvInput &in, int n, int y, int x, int h, int w, int c) {
+  bf16x8 z = {};
+  if (y < 0 || y >= h || x < 0 || x >= w) return z;
+  if (c < in.ca) {
+    if (in.a_up) {
+      const int hh = h >> 1, ww = w >> 1;
+      return *(const bf16x8 *)(in.a + (((size_t)n * hh + (y >> 1)) * ww + (x >> 1)) * in.ca + c);
+    }
+    return *(const bf16x8 *)(in.a + (((size_t)n * h + y) * w + x) * in.ca + c);
+  }
+  return *(const bf16x8 *)(in.b + (((size_t)n * h + y) * w + x) * in.cb + (c - in.ca));
+}
+
+// Forward 3x3 convolution (stride 1, zero padding 1) + bias + ReLU, and the
+// same kernel with flipped/transposed weights for the input gradient.  A
+// workgroup is 4 waves on 4 consecutive rows; a wave computes 64 pixels of
+// its row (4 blocks of 16) for all output channels (MB blocks of 16).
+// Output channels [0, split) go to out0, [split, cout) to out1 (the two
+// halves of a concatenated input when computing its gradient); an optional
+// mask zeroes outputs whose mask activation is <= 0 (the ReLU derivative of
+// the tensor the gradient flows into).
+template <int MB, int PB> __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = blockIdx.z, y = blockIdx.y * 4 + wave, x0 = blockIdx.x * (PB * 16);
+  if (y >= a.h) return;
+  const int col = lane & 15, kg = lane >> 4;
+  const ConvInput in{a.a, a.ca, a.a_up, a.b, a.cb};
+  const int cin = a.ca + a.cb, chunks = cin >> 5;
+  f32x4 acc[MB][PB];
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+    for (int pb = 0; pb < PB; pb++) acc[mb][pb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 9; tap++) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    for (int cc = 0; cc < chunks; cc++) {
+      bf16x8 bx[PB];
+#pragma unroll
+      for (int pb = 0; pb < PB; pb++) bx[pb] = load8(in, n, y + dy, x0 + pb * 16 + col + dx, a.h, a.w, cc * 32 + kg * 8);
+      const bf16 *wrow = a.wt + ((size_t)(tap * chunks + cc) * a.cout + col) * 32 + kg * 8;
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
+        const bf16x8 aw = *(const bf16x8 *)(wrow + (size_t)mb * 16 * 32);
+#pragma unroll
+        for (int pb = 0; pb < PB; pb++) acc[mb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw, bx[pb], acc[mb][pb], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int pb = 0; pb < PB; pb++) {
+    const int x = x0 + pb * 16 + col;
+    if (x >= a.w) continue;
+    const size_t pix = ((size_t)n * a.h + y) * a.w + x;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+      const int co = mb * 16 + kg * 4;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        v[i] = acc[mb][pb][i];
+        if (a.bias) v[i] += a.bias[co + i];
+        if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
+      }
+      bf16 *dst;
+      const bf16 *mask;
+      int stride, c;
+      if (co < a.split) {
+        dst = a.out0;
+        mask = a.mask0;
+        stride = a.split;
+        c = co;
+      } else {
+        dst = a.out1;
+        mask = a.mask1;
+        stride = a.cout - a.split;
+        c = co - a.split;
+      }
+      if (mask) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (!(bf2f(mask[pix * stride + c + i]) > 0.f)) v[i] = 0.f;
+      }
+      typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+      bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
+      *(bf16x4 *)(dst + pix * stride + c) = o;
+    }
+  }
+}
+
+// Weight and bias gradient of a 3x3 convolution:
+//   dW[tap][ci][co] += sum_p X[p + shift(tap)][ci] * dY[p][co],  db[co] += sum_p dY[p][co]
+// One workgroup: one image, one 32-channel group of the input, a 64-pixel
+// column band and `rows` rows.  Per row, the three shifted input rows and the
+// output-gradient row are staged in LDS transposed (channel-major) so that
+// MFMA operands are 8 consecutive pixels of one channel; K runs over pixels.
+// Tiles (tap, co block, ci block) are spread over the 4 waves; partial sums
+// are added to the FP32 gradient with atomics at the end.
+template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const WgradArgs a) {
+  constexpr int XS = 64 + 16;  // staged x range [x0 - 8, x0 + 72): keeps dx = 0 reads 16-byte aligned
+  __shared__ __align__(16) bf16 xs[3][32][XS];
+  __shared__ __align__(16) bf16 ds[MB * 16][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kg = lane >> 4;
+  const int groups = (a.ca + a.cb) >> 5;
+  const int n = blockIdx.z / groups, g = blockIdx.z % groups;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * a.rows;
+  const ConvInput in{a.a, a.ca, a.a_up, a.b, a.cb};
+  // tiles owned by this wave: t = wave, wave+4, ... over (tap, mb, cb) with cb in {0,1}
+  constexpr int NT = 9 * MB * 2;
+  constexpr int TPW = (NT + 3) / 4;
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int cout = MB * 16;
+  for (int r = 0; r < a.rows; r++) {
+    const int y = y0 + r;
+    if (y >= a.h) break;  // uniform across the workgroup
+    // stage input rows y-1, y, y+1 (32 channels of group g), transposed
+    for (int e = threadIdx.x; e < 3 * XS * 4; e += 256) {
+      const int q = e & 3, px = (e >> 2) % XS, dy = (e >> 2) / XS;
+      bf16x8 v = load8(in, n, y + dy - 1, x0 - 8 + px, a.h, a.w, g * 32 + q * 8);
+#pragma unroll
+      for (int i = 0; i < 8; i++) xs[dy][q * 8 + i][px] = v[i];
+    }
+    // stage the output-gradient row, transposed
+    for (int e = threadIdx.x; e < 64 * (cout / 8); e += 256) {
+      const int q = e % (cout / 8), px = e / (cout / 8);
+      const int x = x0 + px;
+      bf16x8 v = {};
+      if (x < a.w) v = *(const bf16x8 *)(a.dy + (((size_t)n * a.h + y) * a.w + x) * cout + q * 8);
+#pragma unroll
+      for (int i = 0; i < 8; i++) ds[q * 8 + i][px] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ti = 0; ti < TPW; ti++) {
+      const int t = wave + ti * 4;
+      if (t < NT) {
+      const int tap = t / (MB * 2), rem = t % (MB * 2), mb = rem >> 1, cb = rem & 1;
+      const int dy = tap / 3, dx = tap % 3 - 1;
+#pragma unroll
+      for (int kc = 0; kc < 2; kc++) {
+        const int p0 = kc * 32 + kg * 8;
+        const bf16x8 av = *(const bf16x8 *)&ds[mb * 16 + col][p0];  // A: co x pixels
+        bf16x8 bv;                                                    // B: pixels x ci
+        const bf16 *src = &xs[dy][cb * 16 + col][p0 + 8 + dx];
+        if (dx == 0) {
+          bv = *(const bf16x8 *)src;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 8; i++) bv[i] = src[i];
+        }
+        acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[ti], 0, 0, 0);
+      }
+      }
+    }
+    __syncthreads();
+  }
+  const int cin = a.ca + a.cb;
+#pragma unroll
+  for (int ti = 0; ti < TPW; ti++) {
+    const int t = wave + ti * 4;
+    if (t < NT) {
+      const int tap = t / (MB * 2), rem = t % (MB * 2), mb = rem >> 1, cb = rem & 1;
+      const int ci = g * 32 + cb * 16 + col;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int co = mb * 16 + kg * 4 + i;
+        atomicAdd(&a.dw[((size_t)tap * cin + ci) * cout + co], acc[ti][i]);
+      }
+    }
+  }
+}
+
+// Per-channel sums of an NHWC tensor (bias gradients), atomically added.
+__global__ __launch_bounds__(256) void channel_sum_kernel(const bf16 *x, int c, size_t npix, float *out) {
+  // thread t sums channel t % c over pixels (t / c), (t / c) + stride, ...
+  const int ch = threadIdx.x % c, lanes_per = 256 / c;
+  const size_t p0 = (size_t)blockIdx.x * lanes_per + threadIdx.x / c;
+  const size_t step = (size_t)gridDim.x * lanes_per;
+  if ((int)threadIdx.x >= lanes_per * c) return;
+  float s = 0.f;
+  for (size_t p = p0; p < npix; p += step) s += bf2f(x[p * c + ch]);
+  atomicAdd(&out[ch], s);
+}
+
+// ------------------------------------------------------------ glue kernels
+
+// Network input: [noisy irradiance rgb, depth, oct normal xy, albedo rgb, 0...]
+// from a compact FP32 record of 12 floats per pixel (the last 3 unused).
+__global__ __launch_bounds__(256) void pack_input_kernel(const float *x12, size_t npix, bf16 *x32) {
+  const size_t p = (size_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int q = threadIdx.x & 3;  // 8-channel quarter
+  if (p >= npix) return;
+  bf16x8 v = {};
+  if (q == 0) {
+    for (int i = 0; i < 8; i++) v[i] = f2bf(x12[p * 12 + i]);
+  } else if (q == 1) {
+    v[0] = f2bf(x12[p * 12 + 8]);
+  }
+  *(bf16x8 *)(x32 + p * 32 + q * 8) = v;
+}
+
+// The same record straight from a G-buffer and the noisy irradiance image.
+template <int NFMT, int AFMT>
+__global__ __launch_bounds__(256) void pack_gbuffer_kernel(const float4 *noisy, int np_, const float *depth, int dp,
+                                                           const void *normal, int nrmp, const void *albedo, int ap,
+                                                           int w, int h, float *x12) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  const float4 c = noisy[(size_t)y * np_ + x];
+  const float4 nn = Texel<NFMT>::load(normal, (size_t)y * nrmp + x);
+  const float4 al = Texel<AFMT>::load(albedo, (size_t)y * ap + x);
+  float *o = x12 + ((size_t)y * w + x) * 12;
+  o[0] = c.x;
+  o[1] = c.y;
+  o[2] = c.z;
+  o[3] = depth[(size_t)y * dp + x];
+  o[4] = nn.x;
+  o[5] = nn.y;
+  o[6] = al.x;
+  o[7] = al.y;
+  o[8] = al.z;
+  o[9] = o[10] = o[11] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void avgpool_kernel(const bf16 *x, int n, int h, int w, int c, bf16 *y) {
+  // y[n][h/2][w/2][c] = mean of the 2x2 children
+  const int hh = h >> 1, ww = w >> 1;
+  const size_t total = (size_t)n * hh * ww * (c / 8);
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+    const int q = (int)(e % (c / 8));
+    size_t p = e / (c / 8);
+    const int xx = (int)(p % ww);
+    p /= ww;
+    const int yy = (int)(p % hh);
+    const int nn = (int)(p / hh);
+    const size_t b00 = (((size_t)nn * h + 2 * yy) * w + 2 * xx) * c + q * 8;
+    const bf16x8 v0 = *(const bf16x8 *)(x + b00), v1 = *(const bf16x8 *)(x + b00 + c);
+    const bf16x8 v2 = *(const bf16x8 *)(x + b00 + (size_t)w * c), v3 = *(const bf16x8 *)(x + b00 + (size_t)w * c + c);
+    bf16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = f2bf((bf2f(v0[i]) + bf2f(v1[i]) + bf2f(v2[i]) + bf2f(v3[i])) * 0.25f);
+    *(bf16x8 *)(y + (((size_t)nn * hh + yy) * ww + xx) * c + q * 8) = o;
+  }
+}
+
+// dx[p] = mask(act[p] > 0) * (dx_in[p] + dpool[p/2] / 4): the gradient of a
+// tensor feeding both a concatenation (dx_in) and an average pool (dpool).
+__global__ __launch_bounds__(256) void pool_bwd_add_kernel(const bf16 *dx_in, const bf16 *dpool, const bf16 *act, int n,
+                                                           int h, int w, int c, bf16 *dx) {
+  const size_t total = (size_t)n * h * w * (c / 8);
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+    const int q = (int)(e % (c / 8));
+    size_t p = e / (c / 8);
+    const int xx = (int)(p % w);
+    p /= w;
+    const int yy = (int)(p % h);
+    const int nn = (int)(p / h);
+    const size_t i0 = (((size_t)nn * h + yy) * w + xx) * c + q * 8;
+    const size_t ip = (((size_t)nn * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1)) * c + q * 8;
+    const bf16x8 a = *(const bf16x8 *)(dx_in + i0), b = *(const bf16x8 *)(dpool + ip), m = *(const bf16x8 *)(act + i0);
+    bf16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = bf2f(m[i]) > 0.f ? f2bf(bf2f(a[i]) + 0.25f * bf2f(b[i])) : f2bf(0.f);
+    *(bf16x8 *)(dx + i0) = o;
+  }
+}
+
+// Gradient through a nearest x2 upsample: dhalf[q] = mask(act[q] > 0) * sum of its 4 children.
+__global__ __launch_bounds__(256) void upsample_bwd_kernel(const bf16 *dfull, const bf16 *act, int n, int hh, int ww, int c,
+                                                           bf16 *dhalf) {
+  const int h = hh * 2, w = ww * 2;
+  const size_t total = (size_t)n * hh * ww * (c / 8);
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+    const int q = (int)(e % (c / 8));
+    size_t p = e / (c / 8);
+    const int xx = (int)(p % ww);
+    p /= ww;
+    const int yy = (int)(p % hh);
+    const int nn = (int)(p / hh);
+    const size_t b00 = (((size_t)nn * h + 2 * yy) * w + 2 * xx) * c + q * 8;
+    const bf16x8 v0 = *(const bf16x8 *)(dfull + b00), v1 = *(const bf16x8 *)(dfull + b00 + c);
+    const bf16x8 v2 = *(const bf16x8 *)(dfull + b00 + (size_t)w * c);
+    const bf16x8 v3 = *(const bf16x8 *)(dfull + b00 + (size_t)w * c + c);
+    const size_t oi = (((size_t)nn * hh + yy) * ww + xx) * c + q * 8;
+    const bf16x8 m = *(const bf16x8 *)(act + oi);
+    bf16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      o[i] = bf2f(m[i]) > 0.f ? f2bf(bf2f(v0[i]) + bf2f(v1[i]) + bf2f(v2[i]) + bf2f(v3[i])) : f2bf(0.f);
+    *(bf16x8 *)(dhalf + oi) = o;
+  }
+}
+
+// Output head: out = noisy + W_h * d1 + b_h (1x1, 32 -> 3, FP32 output RGBA).
+__global__ __launch_bounds__(256) void head_fwd_kernel(const bf16 *d1, const float *x12, const float *wh,
+                                                       const float *bh, size_t npix, float4 *out) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  float o[3];
+  for (int c = 0; c < 3; c++) o[c] = x12[p * 12 + c] + bh[c];
+  for (int q = 0; q < 4; q++) {
+    const bf16x8 v = *(const bf16x8 *)(d1 + p * 32 + q * 8);
+    for (int i = 0; i < 8; i++) {
+      const float f = bf2f(v[i]);
+      for (int c = 0; c < 3; c++) o[c] += wh[c * 32 + q * 8 + i] * f;
+    }
+  }
+  out[p] = make_float4(o[0], o[1], o[2], 1.f);
+}
+
+// L1 loss head backward: g = sign(out - target) / count per channel;
+// dW_h, db_h accumulated per workgroup then atomically; dd1 = mask(d1 > 0) W_h^T g.
+__global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const float4 *out, const float4 *target,
+                                                       const float *wh, size_t npix, float inv_count, bf16 *dd1,
+                                                       float *dwh, float *dbh, float *loss) {
+  __shared__ float sw[96 + 3 + 1];
+  for (int i = threadIdx.x; i < 100; i += 256) sw[i] = 0.f;
+  __syncthreads();
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  float g[3] = {0.f, 0.f, 0.f};
+  float l = 0.f;
+  if (p < npix) {
+    const float4 o = out[p], t = target[p];
+    const float df[3] = {o.x - t.x, o.y - t.y, o.z - t.z};
+    for (int c = 0; c < 3; c++) {
+      g[c] = (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f)) * inv_count;
+      l += fabsf(df[c]);
+    }
+    for (int q = 0; q < 4; q++) {
+      const bf16x8 v = *(const bf16x8 *)(d1 + p * 32 + q * 8);
+      bf16x8 o8;
+      for (int i = 0; i < 8; i++) {
+        const int k = q * 8 + i;
+        const float f = bf2f(v[i]);
+        float s = 0.f;
+        for (int c = 0; c < 3; c++) {
+          s += wh[c * 32 + k] * g[c];
+          atomicAdd(&sw[c * 32 + k], g[c] * f);
+        }
+        o8[i] = f > 0.f ? f2bf(s) : f2bf(0.f);
+      }
+      *(bf16x8 *)(dd1 + p * 32 + q * 8) = o8;
+    }
+    for (int c = 0; c < 3; c++) atomicAdd(&sw[96 + c], g[c]);
+    atomicAdd(&sw[99], l);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 96; i += 256) atomicAdd(&dwh[i], sw[i]);
+  if (threadIdx.x < 3) atomicAdd(&dbh[threadIdx.x], sw[96 + threadIdx.x]);
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, sw[99] * inv_count);
+}
+
+// Adam on FP32 master parameters (Kingma & Ba 2015, bias-corrected).
+__global__ __launch_bounds__(256) void adam_kernel(float *p, const float *g, float *m, float *v, size_t n, float lr,
+                                                   float b1, float b2, float eps, float bc1, float bc2) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float *p, size_t n, float v) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
+// Pack master weights W[co][ci][3][3] (FP32, ci < cin_log) into the forward
+// layout [tap][cin_pad/32][cout][32] and, when wt != null, the input-gradient
+// layout: the same convolution read as cout -> cin_pad with flipped taps,
+// [tap'][cout/32][cin_pad][32] with tap' = 8 - tap.
+__global__ __launch_bounds__(256) void pack_weights_kernel(const float *w, int cout, int cin_log, int cin_pad, bf16 *wf,
+                                                           bf16 *wt) {
+  const size_t total = (size_t)cout * cin_pad * 9;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+    const int tap = (int)(e % 9);
+    const int ci = (int)((e / 9) % cin_pad);
+    const int co = (int)(e / 9 / cin_pad);
+    const float v = ci < cin_log ? w[((size_t)co * cin_log + ci) * 9 + tap] : 0.f;
+    wf[(((size_t)tap * (cin_pad / 32) + ci / 32) * cout + co) * 32 + (ci & 31)] = f2bf(v);
+    if (wt) wt[(((size_t)(8 - tap) * (cout / 32) + co / 32) * cin_pad + ci) * 32 + (co & 31)] = f2bf(v);
+  }
+}
+
+// dW[tap][ci][co] (gradient layout) -> master layout [co][ci][tap] for ci < cin_log.
+__global__ __launch_bounds__(256) void unpack_wgrad_kernel(const float *dw, int cout, int cin_log, int cin_pad,
+                                                           float *g) {
+  const size_t total = (size_t)cout * cin_log * 9;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+    const int tap = (int)(e % 9);
+    const int ci = (int)((e / 9) % cin_log);
+    const int co = (int)(e / 9 / cin_log);
+    g[e] = dw[((size_t)tap * cin_pad + ci) * cout + co];
+  }
+}
+
+// MFMA operand-layout probe: D = A * B for a 16x32 A and 32x16 B given in
+// row-major order, computed by one wave with the layout stated above.
+__global__ __launch_bounds__(64) void mfma_probe_kernel(const float *A, const float *B, float *D) {
+  const int l = threadIdx.x, col = l & 15, kg = l >> 4;
+  bf16x8 a, b;
+  for (int i = 0; i < 8; i++) {
+    a[i] = f2bf(A[col * 32 + kg * 8 + i]);
+    b[i] = f2bf(B[(kg * 8 + i) * 16 + col]);
+  }
+  f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  for (int i = 0; i < 4; i++) D[(kg * 4 + i) * 16 + col] = d[i];
+}
+
+// ------------------------------------------------------------ launchers
+
+static unsigned grid_for(size_t work, size_t per_block) {
+  size_t g = (work + per_block - 1) / per_block;
+  if (g > 65535u * 16u) g = 65535u * 16u;
+  return (unsigned)(g ? g : 1);
+}
+
+// Pixel blocks per wave: 4 while the accumulators fit comfortably, 2 for
+// the wide layers so two waves per SIMD stay resident.
+hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
+  const int mb = a.cout / 16, pb = mb >= 6 ? 2 : 4;
+  dim3 grid((a.w + pb * 16 - 1) / (pb * 16), (a.h + 3) / 4, a.n), block(256);
+  switch (mb) {
+    case 1: hipLaunchKernelGGL((conv3x3_kernel<1, 4>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((conv3x3_kernel<2, 4>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((conv3x3_kernel<4, 4>), grid, block, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((conv3x3_kernel<6, 2>), grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL((conv3x3_kernel<8, 2>), grid, block, 0, s, a); break;
+    case 12: hipLaunchKernelGGL((conv3x3_kernel<12, 2>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s) {
+  const int groups = (a.ca + a.cb) / 32;
+  dim3 grid((a.w + 63) / 64, (a.h + a.rows - 1) / a.rows, a.n * groups), block(256);
+  switch (a.cout / 16) {
+    case 2: hipLaunchKernelGGL(conv3x3_wgrad_kernel<2>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(conv3x3_wgrad_kernel<4>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(conv3x3_wgrad_kernel<8>, grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(channel_sum_kernel, dim3(grid_for(npix, 2048)), dim3(256), 0, s, x, c, npix, out);
+  return hipGetLastError();
+}
+hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s) {
+  hipLaunchKernelGGL(pack_input_kernel, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, s, x12, npix, x32);
+  return hipGetLastError();
+}
+hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12, hipStream_t s) {
+  const int w = noisy.w, h = noisy.h;
+  dim3 grid((w + 63) / 64, (h + 3) / 4), block(256);
+  const float4 *np_ = (const float4 *)noisy.ptr;
+  const float *dp = (const float *)g.depth.ptr;
+  if (g.normal.format == LGX_RG16F && g.albedo.format == LGX_RGBA8)
+    hipLaunchKernelGGL((pack_gbuffer_kernel<LGX_RG16F, LGX_RGBA8>), grid, block, 0, s, np_, noisy.pitch, dp,
+                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x12);
+  else if (g.normal.format == LGX_RG32F && g.albedo.format == LGX_RGBA32F)
+    hipLaunchKernelGGL((pack_gbuffer_kernel<LGX_RG32F, LGX_RGBA32F>), grid, block, 0, s, np_, noisy.pitch, dp,
+                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x12);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t avgpool(const bf16 *x, int n, int h, int w, int c, bf16 *y, hipStream_t s) {
+  hipLaunchKernelGGL(avgpool_kernel, dim3(grid_for((size_t)n * (h / 2) * (w / 2) * (c / 8), 256)), dim3(256), 0, s, x,
+                     n, h, w, c, y);
+  return hipGetLastError();
+}
+hipError_t pool_bwd_add(const bf16 *dx_in, const bf16 *dpool, const bf16 *act, int n, int h, int w, int c, bf16 *dx,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(pool_bwd_add_kernel, dim3(grid_for((size_t)n * h * w * (c / 8), 256)), dim3(256), 0, s, dx_in,
+                     dpool, act, n, h, w, c, dx);
+  return hipGetLastError();
+}
+hipError_t upsample_bwd(const bf16 *dfull, const bf16 *act, int n, int hh, int ww, int c, bf16 *dhalf, hipStream_t s) {
+  hipLaunchKernelGGL(upsample_bwd_kernel, dim3(grid_for((size_t)n * hh * ww * (c / 8), 256)), dim3(256), 0, s, dfull,
+                     act, n, hh, ww, c, dhalf);
+  return hipGetLastError();
+}
+hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, float4 *out,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(head_fwd_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, d1, x12, wh, bh, npix, out);
+  return hipGetLastError();
+}
+hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
+                    float inv_count, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s) {
+  hipLaunchKernelGGL(head_bwd_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, d1, out, target, wh, npix,
+                     inv_count, dd1, dwh, dbh, loss);
+  return hipGetLastError();
+}
+hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                float bc1, float bc2, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2);
+  return hipGetLastError();
+}
+hipError_t fill(float *p, size_t n, float v, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t pack_weights(const float *w, int cout, int cin_log, int cin_pad, bf16 *wf, bf16 *wt, hipStream_t s) {
+  hipLaunchKernelGGL(pack_weights_kernel, dim3(grid_for((size_t)cout * cin_pad * 9, 256)), dim3(256), 0, s, w, cout,
+                     cin_log, cin_pad, wf, wt);
+  return hipGetLastError();
+}
+hipError_t unpack_wgrad(const float *dw, int cout, int cin_log, int cin_pad, float *g, hipStream_t s) {
+  hipLaunchKernelGGL(unpack_wgrad_kernel, dim3(grid_for((size_t)cout * cin_log * 9, 256)), dim3(256), 0, s, dw, cout,
+                     cin_log, cin_pad, g);
+  return hipGetLastError();
+}
+hipError_t mfma_probe(const float *A, const float *B, float *D, hipStream_t s) {
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, A, B, D);
+  return hipGetLastError();
+}
+
+}  // namespace unet
+}  // namespace lgx

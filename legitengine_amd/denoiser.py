@@ -1,0 +1,196 @@
+"""Learned denoiser: training pairs from the project's own gather, the
+training loop on the BF16 MFMA U-Net, and evaluation against the
+256-direction FP64 gather.
+
+Pairs: a seeded scene (32 boxes, 8 spheres) seen from a seeded camera pose
+at 1920x1080; a 256x256 crop (origin on the 4-pixel interleave grid, so the
+pattern phase matches full frames) of the 4-direction gather is the input,
+the 256-direction gather accumulated in FP64 on the same crop the target.
+Run: python -m legitengine_amd.denoiser train --scenes N --steps K --out PATH
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import device as dev
+from . import lgx
+
+W, H = 1920, 1080
+LEVELS = 6
+CROP = 256
+NOISY_DIRS, TARGET_DIRS, PLAIN_DIRS, STEPS = 4, 256, 32, 64
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_1080p.safetensors")
+
+
+class FrameGen:
+    """Device resources to render a seeded view and gather on it."""
+
+    def __init__(self, w=W, h=H, levels=LEVELS):
+        self.w, self.h = w, h
+        self.g = dev.GBufferDev.alloc(w, h, "mixed")
+        self.pyr = dev.PyramidDev(self.g, levels, lgx.SAMPLER_HIZ_POINT)
+        self.gathers = {}
+        self.out32 = dev.DImage(w, h, lgx.RGBA32F)
+        self.out64 = dev.DImage(w, h, lgx.RGBA64F)
+
+    def gather(self, dirs, steps=STEPS, precision=lgx.PREC_F32):
+        key = (dirs, steps, precision)
+        if key not in self.gathers:
+            self.gathers[key] = dev.Gather(self.w, self.h, dirs, steps, lgx.SAMPLER_HIZ_POINT, precision=precision)
+        return self.gathers[key]
+
+    def view(self, scene_seed, pose=0, boxes=32, spheres=8):
+        sc = lgx.make_scene(scene_seed, boxes, spheres)
+        if pose:
+            lgx.scene_pose(sc, (scene_seed << 8) | pose)
+        view, proj = lgx.scene_matrices(sc, self.w, self.h)
+        self.g.render(sc)
+        self.pyr.build()
+        return view, proj
+
+    def records(self, noisy, y0=0, x0=0, h=None, w=None):
+        """x12 records [h, w, 12] of a region from the G-buffer and a noisy image."""
+        h = h or self.h
+        w = w or self.w
+        sl = (slice(y0, y0 + h), slice(x0, x0 + w))
+        rec = torch.zeros((h, w, 12), dtype=torch.float32, device="cuda")
+        rec[..., 0:3] = noisy.t[sl][..., :3]
+        rec[..., 3] = self.g.depth.t[sl]
+        rec[..., 4:6] = self.g.normal.t[sl].float()
+        rec[..., 6:9] = self.g.albedo.t[sl][..., :3].float() * np.float32(1.0 / 255.0)
+        return rec
+
+
+def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx.PREC_F32_ACC64):
+    fg = FrameGen()
+    n = n_scenes * poses
+    X = torch.empty((n, CROP, CROP, 12), dtype=torch.float32, device="cuda")
+    T = torch.empty((n, CROP, CROP, 4), dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(seed0)
+    noisy_g = fg.gather(NOISY_DIRS)
+    target_g = fg.gather(TARGET_DIRS, 0, target_precision)
+    t0 = time.time()
+    last = t0
+    for i in range(n):
+        s, p = seed0 + i // poses, i % poses
+        view, proj = fg.view(s, p)
+        x0 = int(rng.integers(0, (W - CROP) // 4 + 1)) * 4
+        y0 = int(rng.integers(0, (H - CROP) // 4 + 1)) * 4
+        roi = (x0, y0, CROP, CROP)
+        noisy_g(view, proj, fg.g, fg.pyr, out=fg.out32, roi=roi)
+        target_g(view, proj, fg.g, fg.pyr, out=fg.out64, roi=roi)
+        X[i] = fg.records(fg.out32, y0, x0, CROP, CROP)
+        T[i] = fg.out64.t[y0:y0 + CROP, x0:x0 + CROP].float()
+        if time.time() - last > 20 or i == n - 1:
+            torch.cuda.synchronize()
+            last = time.time()
+            log(f"dataset {i + 1}/{n} views, {last - t0:.1f} s")
+    return X, T
+
+
+def train(net, X, T, steps, batch=16, lr=1e-3, log=print, seed=0, log_every=50):
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    hist = []
+    t0 = time.time()
+    torch.cuda.synchronize()
+    for it in range(steps):
+        idx = torch.randint(0, X.shape[0], (batch,), device="cuda", generator=gen)
+        xb = X.index_select(0, idx).contiguous()
+        tb = T.index_select(0, idx).contiguous()
+        loss.zero_()
+        net.train_step(xb, tb, lr, loss)
+        if it % log_every == 0 or it == steps - 1:
+            hist.append((it, float(loss.item())))
+            log(f"step {it} loss {hist[-1][1]:.5f} {time.time() - t0:.1f} s")
+    torch.cuda.synchronize()
+    return hist, time.time() - t0
+
+
+def rel_l2(a, b):
+    a = a[..., :3].double()
+    b = b[..., :3].double()
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def evaluate(net, seeds, log=print, plain_dirs=(4, 8, 16, 32)):
+    """rel-L2 of the denoised 4-direction path and of plain paths against the
+    256-direction FP64 gather on full 1080p frames."""
+    fg = FrameGen()
+    res = []
+    for s in seeds:
+        view, proj = fg.view(s, 0)
+        fg.gather(TARGET_DIRS, 0, lgx.PREC_F64)(view, proj, fg.g, fg.pyr, out=fg.out64)
+        tgt = fg.out64.t.clone()
+        row = {"seed": s}
+        fg.gather(NOISY_DIRS)(view, proj, fg.g, fg.pyr, out=fg.out32)
+        row["noisy4"] = rel_l2(fg.out32.t, tgt)
+        den = net.denoise(fg.g, fg.out32)
+        row["denoised4"] = rel_l2(den.t, tgt)
+        for d in plain_dirs:
+            o = fg.gather(d)(view, proj, fg.g, fg.pyr, out=fg.out32)
+            b = o
+            for _ in range(2):
+                b = dev.bilateral(b, fg.g, radius=4)
+            row[f"plain{d}"] = rel_l2(b.t, tgt)
+            row[f"raw{d}"] = rel_l2(o.t, tgt)
+        torch.cuda.synchronize()
+        log(json.dumps(row))
+        res.append(row)
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--scenes", type=int, default=500)
+    t.add_argument("--poses", type=int, default=4)
+    t.add_argument("--steps", type=int, default=2000)
+    t.add_argument("--batch", type=int, default=16)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--out", default=WEIGHTS)
+    t.add_argument("--resume", default=None)
+    t.add_argument("--eval-seeds", default="200,201")
+    t.add_argument("--report", default=None)
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", default=WEIGHTS)
+    e.add_argument("--seeds", default="200,201,202,203,204,205,206,207")
+    args = ap.parse_args(argv)
+    dev.require_device()
+
+    def log(msg):
+        print(msg, flush=True)
+
+    if args.cmd == "train":
+        X, T = build_dataset(args.scenes, args.poses, log=log)
+        net = dev.UNet(args.batch, CROP, CROP, seed=0)
+        if args.resume:
+            net.load(args.resume)
+        hist, secs = train(net, X, T, args.steps, args.batch, args.lr, log=log)
+        os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+        net.save(args.out)
+        log(f"saved {args.out}: {args.steps} steps in {secs:.1f} s ({args.steps / secs:.2f} steps/s)")
+        del X, T
+        infer = dev.UNet(1, H, W)
+        infer.load(args.out)
+        rows = evaluate(infer, [int(s) for s in args.eval_seeds.split(",") if s])
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump({"history": hist, "train_seconds": secs, "steps": args.steps, "batch": args.batch,
+                           "scenes": args.scenes, "poses": args.poses, "eval": rows}, f)
+    else:
+        net = dev.UNet(1, H, W)
+        net.load(args.weights)
+        evaluate(net, [int(s) for s in args.seeds.split(",") if s])
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

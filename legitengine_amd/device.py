@@ -143,10 +143,11 @@ class Gather:
             raise lgx.LgxError(lgx.EINVAL, "lgx_gather_workspace_bytes")
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         lgx.check(lgx.lib().lgx_gather_prepare(C.byref(p), C.c_void_p(self.ws.data_ptr())), "lgx_gather_prepare")
-        self.out_fmt = lgx.RGBA64F if precision == lgx.PREC_F64 else lgx.RGBA32F
+        self.out_fmt = lgx.RGBA32F if precision == lgx.PREC_F32 else lgx.RGBA64F
 
-    def __call__(self, view, proj, gb, pyr, out=None, step_counts=None, stream=None):
+    def __call__(self, view, proj, gb, pyr, out=None, step_counts=None, stream=None, roi=None):
         p = self.p
+        p.roi[:] = list(roi) if roi is not None else [0, 0, 0, 0]
         p.view[:] = [float(v) for v in view]
         p.proj[:] = [float(v) for v in proj]
         p.step_counts = C.c_void_p(step_counts.data_ptr()) if step_counts is not None else None
@@ -243,3 +244,96 @@ class Renderer:
             self.close()
         except Exception:
             pass
+
+
+class UNet:
+    """The BF16 MFMA U-Net denoiser (lgx_unet_*)."""
+
+    def __init__(self, max_n, max_h, max_w, seed=0):
+        h = C.c_void_p()
+        lgx.check(lgx.lib().lgx_unet_create(max_n, max_h, max_w, C.c_uint64(seed), C.byref(h)), "lgx_unet_create")
+        self.h = h
+        self.n_params = int(lgx.lib().lgx_unet_param_count())
+
+    def forward(self, x12, out=None, stream=None):
+        n, h, w, _ = x12.shape
+        if out is None:
+            out = torch.empty((n, h, w, 4), dtype=torch.float32, device="cuda")
+        lgx.check(lgx.lib().lgx_unet_forward(self.h, C.c_void_p(x12.data_ptr()), n, h, w, C.c_void_p(out.data_ptr()),
+                                             stream_handle(stream)), "lgx_unet_forward")
+        return out
+
+    def backward(self, x12, target, loss=None, stream=None):
+        n, h, w, _ = x12.shape
+        lp = C.c_void_p(loss.data_ptr()) if loss is not None else None
+        lgx.check(lgx.lib().lgx_unet_backward(self.h, C.c_void_p(x12.data_ptr()), C.c_void_p(target.data_ptr()), n, h,
+                                              w, lp, stream_handle(stream)), "lgx_unet_backward")
+
+    def adam_step(self, lr, stream=None):
+        lgx.check(lgx.lib().lgx_unet_adam_step(self.h, C.c_float(lr), stream_handle(stream)), "lgx_unet_adam_step")
+
+    def train_step(self, x12, target, lr, loss=None, stream=None):
+        n, h, w, _ = x12.shape
+        lp = C.c_void_p(loss.data_ptr()) if loss is not None else None
+        lgx.check(lgx.lib().lgx_unet_train_step(self.h, C.c_void_p(x12.data_ptr()), C.c_void_p(target.data_ptr()), n,
+                                                h, w, C.c_float(lr), lp, stream_handle(stream)), "lgx_unet_train_step")
+
+    def get(self, which=0):
+        a = np.zeros(self.n_params, np.float32)
+        t = C.c_int32()
+        lgx.check(lgx.lib().lgx_unet_get_state(self.h, which, C.c_void_p(a.ctypes.data), self.n_params, C.byref(t)),
+                  "lgx_unet_get_state")
+        return a, t.value
+
+    def set(self, which, a, adam_t=-1):
+        a = np.ascontiguousarray(a, np.float32)
+        lgx.check(lgx.lib().lgx_unet_set_state(self.h, which, C.c_void_p(a.ctypes.data), self.n_params, adam_t),
+                  "lgx_unet_set_state")
+
+    def denoise(self, gb, noisy, out=None, stream=None):
+        if out is None:
+            out = DImage(noisy.w, noisy.h, lgx.RGBA32F)
+        g, i, o = gb.desc, noisy.desc, out.desc
+        lgx.check(lgx.lib().lgx_unet_denoise(self.h, C.byref(g), C.byref(i), C.byref(o), stream_handle(stream)),
+                  "lgx_unet_denoise")
+        return out
+
+    def save(self, path):
+        """Checkpoint: parameters and Adam state (safetensors, no code executed on load)."""
+        from safetensors.numpy import save_file
+        p, t = self.get(0)
+        m, _ = self.get(2)
+        v, _ = self.get(3)
+        save_file({"params": p, "adam_m": m, "adam_v": v, "adam_t": np.array([t], np.int32)}, path)
+
+    def load(self, path):
+        from safetensors.numpy import load_file
+        d = load_file(path)
+        if d["params"].size != self.n_params:
+            raise ValueError(f"checkpoint has {d['params'].size} parameters, the network {self.n_params}")
+        t = int(d["adam_t"][0]) if "adam_t" in d else 0
+        self.set(0, d["params"], t)
+        if "adam_m" in d:
+            self.set(2, d["adam_m"], t)
+            self.set(3, d["adam_v"], t)
+
+    def close(self):
+        if self.h:
+            lgx.lib().lgx_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mfma_probe(A, B):
+    a = torch.from_numpy(np.ascontiguousarray(A, np.float32)).cuda()
+    b = torch.from_numpy(np.ascontiguousarray(B, np.float32)).cuda()
+    d = torch.zeros((16, 16), dtype=torch.float32, device="cuda")
+    lgx.check(lgx.lib().lgx_unet_mfma_probe(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
+                                            C.c_void_p(d.data_ptr()), stream_handle()), "lgx_unet_mfma_probe")
+    torch.cuda.synchronize()
+    return d.cpu().numpy()

@@ -17,7 +17,7 @@ R32F, RG32F, RGBA32F, RG16F, RGBA16F, RGBA8, RGBA64F, R16F = 1, 2, 3, 4, 5, 6, 7
 FORMAT_BYTES = {R32F: 4, RG32F: 8, RGBA32F: 16, RG16F: 4, RGBA16F: 8, RGBA8: 4, RGBA64F: 32, R16F: 2}
 MIP_AVG, MIP_MAX = 0, 1
 SAMPLER_TRILINEAR, SAMPLER_HIZ_POINT = 0, 1
-PREC_F32, PREC_F64 = 0, 1
+PREC_F32, PREC_F64, PREC_F32_ACC64 = 0, 1, 2
 OUT_INTERLEAVED, OUT_SLICES = 0, 1
 OK, EINVAL, EFORMAT, ESIZE, EUNSUPPORTED, EWORKSPACE, ENODEVICE = 0, -1, -2, -3, -4, -5, -6
 
@@ -39,7 +39,7 @@ class GatherParams(C.Structure):
     _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("width", C.c_int32), ("height", C.c_int32),
                 ("dirs", C.c_int32), ("max_steps", C.c_int32), ("grid", C.c_int32), ("sampler", C.c_int32),
                 ("precision", C.c_int32), ("out_layout", C.c_int32), ("step_counts", C.c_void_p),
-                ("tile", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("tile", C.c_int32), ("roi", C.c_int32 * 4)]
 
 
 class BilateralParams(C.Structure):
@@ -87,6 +87,7 @@ SIGNATURES = {
     "lgx_scene_matrices": (C.c_int, [C.POINTER(Scene), C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                      C.POINTER(C.c_float)]),
     "lgx_scene_render_cpu": (C.c_int, [C.POINTER(Scene), C.POINTER(GBuffer)]),
+    "lgx_scene_pose": (C.c_int, [C.POINTER(Scene), C.c_uint64]),
     "lgx_scene_render_gpu": (C.c_int, [C.POINTER(Scene), C.POINTER(GBuffer), C.c_void_p]),
     "lgx_renderer_create": (C.c_int, [C.POINTER(RendererConfig), C.POINTER(C.c_void_p)]),
     "lgx_renderer_render": (C.c_int, [C.c_void_p, C.POINTER(GBuffer), C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -99,6 +100,20 @@ SIGNATURES = {
     "lgx_renderer_last_error": (C.c_char_p, [C.c_void_p]),
     "lgx_renderer_destroy": (None, [C.c_void_p]),
     "lgx_copy_image": (C.c_int, [C.POINTER(Image), C.POINTER(Image), C.c_void_p]),
+    "lgx_unet_param_count": (C.c_int64, []),
+    "lgx_unet_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "lgx_unet_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "lgx_unet_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                    C.c_void_p]),
+    "lgx_unet_adam_step": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
+    "lgx_unet_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                      C.c_void_p, C.c_void_p]),
+    "lgx_unet_output": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "lgx_unet_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]),
+    "lgx_unet_set_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32]),
+    "lgx_unet_denoise": (C.c_int, [C.c_void_p, C.POINTER(GBuffer), C.POINTER(Image), C.POINTER(Image), C.c_void_p]),
+    "lgx_unet_mfma_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lgx_unet_destroy": (None, [C.c_void_p]),
     "lgx_strerror": (C.c_char_p, [C.c_int]),
     "lgx_version": (C.c_int32, []),
 }
@@ -155,6 +170,12 @@ def make_scene(seed, n_boxes, n_spheres):
     s = Scene()
     check(lib().lgx_scene_make(C.c_uint64(seed), n_boxes, n_spheres, C.byref(s)), "lgx_scene_make")
     return s
+
+
+def scene_pose(scene, pose_seed):
+    """Re-samples the camera of `scene` (same geometry) from pose_seed, in place."""
+    check(lib().lgx_scene_pose(C.byref(scene), C.c_uint64(pose_seed)), "lgx_scene_pose")
+    return scene
 
 
 def scene_matrices(scene, w, h):

@@ -262,3 +262,32 @@ def test_rejects_bad_arguments(dev):
     p = lgx.GatherParams()
     p.width, p.height, p.dirs, p.grid = 64, 48, 0, 4
     assert L.lgx_gather_workspace_bytes(C.byref(p)) == 0
+
+
+def test_gather_roi_matches_full_frame(dev, c1):
+    sc, view, proj, gb = c1[0]
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, 5, lgx.SAMPLER_HIZ_POINT).build()
+    ga = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT)
+    full = ga(view, proj, g, pyr).numpy()
+    out = dev.DImage(g.w, g.h, lgx.RGBA32F)
+    out.t.fill_(-7.0)
+    counts = torch.zeros((g.h, g.w), dtype=torch.int32, device="cuda")
+    ga(view, proj, g, pyr, out=out, step_counts=counts, roi=(37, 90, 101, 64))
+    torch.cuda.synchronize()
+    o = out.numpy()
+    assert o[90:154, 37:138].tobytes() == full[90:154, 37:138].tobytes()
+    mask = np.ones(o.shape[:2], bool)
+    mask[90:154, 37:138] = False
+    assert (o[mask] == -7.0).all()  # pixels outside the region are not written
+    assert (counts.cpu().numpy()[mask] == 0).all()
+
+
+def test_gather_f32_acc64_vs_f64_oracle(dev, c1):
+    sc, view, proj, gb = c1[1]
+    g, pyr, out, counts = _gpu_gather(dev, gb, view, proj, 5, 64, 0, lgx.SAMPLER_HIZ_POINT, lgx.PREC_F32_ACC64)
+    dep_b, rad_b, _, _ = _pyr_host(gb, 5, lgx.SAMPLER_HIZ_POINT)
+    ref = orc.gather(view, proj, gb["depth"], orc.widen(gb["normal"]), dep_b, rad_b, 64, 0, 1, precision="f64",
+                     crop=(64, 64, 48, 48))
+    assert out.dtype == np.float64
+    assert orc.rel_l2(out[64:112, 64:112], ref) <= 2e-4

@@ -207,7 +207,7 @@ int lgx_scene_render_gpu(const lgx_scene *s, const lgx_gbuffer *g, lgx_stream_t 
  * (RecreateSwapchainResources, SSVGIRenderer.h:24-29); frames allocate
  * nothing.  The G-buffer is caller memory, rebound each frame. */
 typedef struct lgx_renderer lgx_renderer;
-enum { LGX_DENOISER_NONE = 0, LGX_DENOISER_BILATERAL = 1, LGX_DENOISER_REGRESSION = 2 };
+enum { LGX_DENOISER_NONE = 0, LGX_DENOISER_BILATERAL = 1, LGX_DENOISER_REGRESSION = 2, LGX_DENOISER_UNET = 3 };
 typedef struct {
   int32_t width, height;
   int32_t dirs, max_steps, levels, grid, sampler;
@@ -230,6 +230,10 @@ int lgx_renderer_render(lgx_renderer *r, const lgx_gbuffer *g, const float view[
 int lgx_renderer_capture(lgx_renderer *r, const lgx_gbuffer *g, const float view[16], const float proj[16],
                          lgx_stream_t stream);
 int lgx_renderer_replay(lgx_renderer *r, lgx_stream_t stream);
+/* LGX_DENOISER_UNET: the network the DenoiserPass runs (created with max n >= 1
+ * and the viewport extent; owned by the caller, must outlive the renderer). */
+struct lgx_unet;
+int lgx_renderer_attach_unet(lgx_renderer *r, struct lgx_unet *u);
 /* Descriptor of a named intermediate ("depth_pyramid", "radiance_pyramid",
  * "blurred_radiance", "blurred_depth", "indirect_slices", "indirect",
  * "denoised", "final") at mip level `mip`. */
@@ -271,7 +275,7 @@ int lgx_unet_output(lgx_unet *u, float **out);
  * layout [conv l: W[co][ci][3][3], b[co]] x 10, head W[3][32], b[3]. */
 int lgx_unet_get_state(lgx_unet *u, int32_t which, float *host, int64_t n, int32_t *adam_t);
 int lgx_unet_set_state(lgx_unet *u, int32_t which, const float *host, int64_t n, int32_t adam_t);
-/* Inference from a G-buffer and the noisy irradiance (RGBA32F); out RGBA32F, unpitched. */
+/* Inference from a G-buffer and the noisy irradiance (RGBA32F); out RGBA32F (any pitch). */
 int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_image *noisy, lgx_image *out, lgx_stream_t stream);
 /* One wave computes D(16x16) = A(16x32, row-major) * B(32x16, row-major) on the
  * matrix core through the operand layout the convolutions use (device FP32

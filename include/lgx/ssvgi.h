@@ -77,6 +77,7 @@ struct SsvgiConfig {
   int32_t normal_format = LGX_RG16F, albedo_format = LGX_RGBA8, radiance_format = LGX_RGBA16F;
   int32_t tile = 0;
   bool composite = false;
+  struct lgx_unet *unet = nullptr;  // LGX_DENOISER_UNET
 };
 
 struct SsvgiFrame {
@@ -100,6 +101,7 @@ class SsvgiRenderer {
   // "denoised", "final".
   lgx_image Intermediate(const char *name, uint32_t mip) const;
   const SsvgiConfig &Config() const { return cfg_; }
+  void AttachUnet(struct lgx_unet *u) { cfg_.unet = u; }
 
  private:
   struct ViewportResources;

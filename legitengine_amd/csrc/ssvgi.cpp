@@ -238,7 +238,28 @@ void SsvgiRenderer::RenderFrame(const SsvgiFrame &f) {
                                 uvec2{(uint32_t)c.grid, (uint32_t)c.grid});
   // :265-302 DenoiserPass
   RenderGraph::ImageViewProxyId cur = v.indirect.imageViewProxy.Id();
-  if (c.denoiser != LGX_DENOISER_NONE) {
+  if (c.denoiser == LGX_DENOISER_UNET) {
+    if (!c.unet) throw Error(LGX_EINVAL, "SsvgiRenderer: LGX_DENOISER_UNET without an attached network");
+    auto depth0 = v.depth.mipImageViewProxies[0].Id();
+    auto normal = v.normal.Id(), albedo = v.albedo.Id();
+    auto dst = v.denoised[0].imageViewProxy.Id();
+    auto src = cur;
+    lgx_unet *net = c.unet;
+    rg_->AddPass(RenderGraph::ComputePassDesc()
+                     .SetInputImages({src, depth0, normal, albedo})
+                     .SetStorageImages({dst})
+                     .SetProfilerInfo(kGreenSea, "DenoiserPass")
+                     .SetRecordFunc([net, src, dst, depth0, normal, albedo](RenderGraph::PassContext ctx) {
+                       lgx_gbuffer gb;
+                       memset(&gb, 0, sizeof(gb));
+                       gb.depth = ctx.GetImageView(depth0);
+                       gb.normal = ctx.GetImageView(normal);
+                       gb.albedo = ctx.GetImageView(albedo);
+                       lgx_image a = ctx.GetImageView(src), o = ctx.GetImageView(dst);
+                       Check(lgx_unet_denoise(net, &gb, &a, &o, ctx.GetStream()), "lgx_unet_denoise");
+                     }));
+    cur = dst;
+  } else if (c.denoiser != LGX_DENOISER_NONE) {
     auto depth0 = v.depth.mipImageViewProxies[0].Id();
     auto normal = v.normal.Id();
     const int passes = c.denoiser == LGX_DENOISER_REGRESSION ? 1 : c.denoise_passes;
@@ -397,6 +418,13 @@ extern "C" int lgx_renderer_capture(lgx_renderer *r, const lgx_gbuffer *g, const
 extern "C" int lgx_renderer_replay(lgx_renderer *r, lgx_stream_t stream) {
   if (!r) return LGX_EINVAL;
   return guard(r, [&]() { r->rg.Replay(stream); });
+}
+
+extern "C" int lgx_renderer_attach_unet(lgx_renderer *r, lgx_unet *u) {
+  if (!r || !u) return LGX_EINVAL;
+  if (r->r->Config().denoiser != LGX_DENOISER_UNET) return LGX_EINVAL;
+  r->r->AttachUnet(u);
+  return LGX_OK;
 }
 
 extern "C" int lgx_renderer_intermediate(lgx_renderer *r, const char *name, int32_t mip, lgx_image *out) {

@@ -47,8 +47,9 @@ hipError_t avgpool(const bf16 *x, int n, int h, int w, int c, bf16 *y, hipStream
 hipError_t pool_bwd_add(const bf16 *dx_in, const bf16 *dpool, const bf16 *act, int n, int h, int w, int c, bf16 *dx,
                         hipStream_t s);
 hipError_t upsample_bwd(const bf16 *dfull, const bf16 *act, int n, int hh, int ww, int c, bf16 *dhalf, hipStream_t s);
-hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, float4 *out,
-                    hipStream_t s);
+// out row pitch in pixels: rows of w pixels (over all n images) start every out_pitch
+hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, int w,
+                    int out_pitch, float4 *out, hipStream_t s);
 hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
                     float inv_count, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s);
 hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,

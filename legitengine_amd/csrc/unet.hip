@@ -216,7 +216,7 @@ class Net {
     check(channel_sum(dy, kLayers[l].cout, (size_t)n * h * w, grads + b_off[l], s), "channel_sum");
   }
 
-  void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s) {
+  void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
     const size_t P = (size_t)n * h * w;
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
     check(pack_input(x12, P, x32, s), "pack_input");
@@ -232,7 +232,9 @@ class Net {
     conv(7, d2a, 64, 0, nullptr, 0, n, h2, w2, d2, s);
     conv(8, d2, 64, 1, e1, 32, n, h, w, d1a, s);
     conv(9, d1a, 32, 0, nullptr, 0, n, h, w, d1, s);
-    check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, (float4 *)out_, s), "head_fwd");
+    check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, w, out_pitch > 0 ? out_pitch : w,
+                   (float4 *)out_, s),
+          "head_fwd");
   }
 
   void backward(const float *x12, const float *target, int n, int h, int w, hipStream_t s) {
@@ -425,15 +427,15 @@ extern "C" int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_ima
                                 lgx_stream_t stream) {
   if (!u || !u->net || !g || !noisy || !out) return LGX_EINVAL;
   if (noisy->format != LGX_RGBA32F || out->format != LGX_RGBA32F) return LGX_EFORMAT;
-  if (out->w != noisy->w || out->h != noisy->h || out->pitch != out->w || g->depth.w != noisy->w ||
-      g->depth.h != noisy->h)
+  if (out->w != noisy->w || out->h != noisy->h || g->depth.w != noisy->w || g->depth.h != noisy->h ||
+      g->normal.w != noisy->w || g->albedo.w != noisy->w)
     return LGX_ESIZE;
   if (!shape_ok(u, 1, noisy->h, noisy->w)) return LGX_ESIZE;
   return unet_guard([&]() {
     Net &N = *u->net;
     hipStream_t s = (hipStream_t)stream;
     N.check(lgx::unet::pack_gbuffer(*noisy, *g, N.x12_scratch, s), "pack_gbuffer");
-    N.forward(N.x12_scratch, 1, noisy->h, noisy->w, (float *)out->ptr, s);
+    N.forward(N.x12_scratch, 1, noisy->h, noisy->w, (float *)out->ptr, s, out->pitch);
   });
 }
 

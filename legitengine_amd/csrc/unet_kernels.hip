@@ -343,7 +343,8 @@ __global__ __launch_bounds__(256) void upsample_bwd_kernel(const bf16 *dfull, co
 
 // Output head: out = noisy + W_h * d1 + b_h (1x1, 32 -> 3, FP32 output RGBA).
 __global__ __launch_bounds__(256) void head_fwd_kernel(const bf16 *d1, const float *x12, const float *wh,
-                                                       const float *bh, size_t npix, float4 *out) {
+                                                       const float *bh, size_t npix, int w, int out_pitch,
+                                                       float4 *out) {
   const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= npix) return;
   float o[3];
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(256) void head_fwd_kernel(const bf16 *d1, const flo
       for (int c = 0; c < 3; c++) o[c] += wh[c * 32 + q * 8 + i] * f;
     }
   }
-  out[p] = make_float4(o[0], o[1], o[2], 1.f);
+  out[(p / w) * out_pitch + p % w] = make_float4(o[0], o[1], o[2], 1.f);
 }
 
 // L1 loss head backward: g = sign(out - target) / count per channel;
@@ -535,9 +536,10 @@ hipError_t upsample_bwd(const bf16 *dfull, const bf16 *act, int n, int hh, int w
                      act, n, hh, ww, c, dhalf);
   return hipGetLastError();
 }
-hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, float4 *out,
-                    hipStream_t s) {
-  hipLaunchKernelGGL(head_fwd_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, d1, x12, wh, bh, npix, out);
+hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, int w,
+                    int out_pitch, float4 *out, hipStream_t s) {
+  hipLaunchKernelGGL(head_fwd_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, d1, x12, wh, bh, npix, w,
+                     out_pitch, out);
   return hipGetLastError();
 }
 hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,

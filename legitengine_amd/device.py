@@ -212,6 +212,10 @@ class Renderer:
         p = (C.c_float * 16)(*[float(x) for x in proj])
         self._err(lgx.lib().lgx_renderer_capture(self.h, C.byref(g), v, p, stream_handle(stream)), "capture")
 
+    def attach_unet(self, net):
+        self._unet = net  # keep the network alive as long as the renderer
+        self._err(lgx.lib().lgx_renderer_attach_unet(self.h, net.h), "attach_unet")
+
     def replay(self, stream=None):
         self._err(lgx.lib().lgx_renderer_replay(self.h, stream_handle(stream)), "replay")
 

@@ -65,7 +65,7 @@ class RendererConfig(C.Structure):
                 ("profile", C.c_int32), ("validate", C.c_int32), ("reserved", C.c_int32)]
 
 
-DENOISER_NONE, DENOISER_BILATERAL, DENOISER_REGRESSION = 0, 1, 2
+DENOISER_NONE, DENOISER_BILATERAL, DENOISER_REGRESSION, DENOISER_UNET = 0, 1, 2, 3
 
 # Every entry point declared in include/lgx.h, with its ctypes signature.
 SIGNATURES = {
@@ -95,6 +95,7 @@ SIGNATURES = {
     "lgx_renderer_capture": (C.c_int, [C.c_void_p, C.POINTER(GBuffer), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                        C.c_void_p]),
     "lgx_renderer_replay": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lgx_renderer_attach_unet": (C.c_int, [C.c_void_p, C.c_void_p]),
     "lgx_renderer_intermediate": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(Image)]),
     "lgx_renderer_pass_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_size_t]),
     "lgx_renderer_last_error": (C.c_char_p, [C.c_void_p]),

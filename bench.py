@@ -11,6 +11,13 @@ Prints ONE JSON line (rank 0) with the metric, the roofline of the dominant
 kernel (the gather; achieved = algorithmic bytes per launch / its average
 launch time measured with HIP events on the launch stream) and the CPU
 baseline (the FP32 oracle, timed on a bounded sample of the same frame).
+
+The "denoised" object holds the other half of the metric: the renderer with
+a 4-direction gather and the trained BF16 U-Net as its denoiser pass, timed
+the same way, and the relative L2 error of its output and of the plain path
+at 32..256 directions against the 256-direction gather accumulated in FP64
+on the GPU (seeded scenes 200..207), with the plain path's ms/frame at the
+denoised path's error (log-log interpolation over the sweep).
 """
 import argparse
 import json
@@ -28,6 +35,12 @@ SEED = 100
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per unit (storage precision, no cache credit)
 BD, BR, BC, BO = 4, 8, 8, 16
+DEN_DIRS, TARGET_DIRS = 4, 256
+PLAIN_SWEEP = (32, 64, 128, 256)
+QUALITY_SEEDS = (200, 201, 202, 203, 204, 205, 206, 207)
+BATCH_SEEDS = range(3000, 3032)
+BATCH_SIZES = ((512, 512), (1280, 720), (1920, 1080), (2560, 1440))
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def parse():
@@ -37,6 +50,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-px", type=int, default=96 * 96)
+    ap.add_argument("--no-denoised", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--quality-seeds", default=",".join(str(s) for s in QUALITY_SEEDS))
+    ap.add_argument("--weights", default=None)
     return ap.parse_args()
 
 
@@ -107,10 +124,193 @@ def main():
         "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(passes)},
     }
 
+    if not args.no_denoised:
+        seeds = [int(s) for s in args.quality_seeds.split(",") if s]
+        result["denoised"] = denoised_leg(args, g, view, proj, sampler, ms_per_step, seeds)
+
+    if not args.no_batch:
+        result["uneven_batch"] = batch_leg(args, sampler)
+
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(g, view, proj, sampler, renderer.intermediate("indirect"),
                                               args.cpu_sample_px)
     print(json.dumps(result), flush=True)
+
+
+def time_renderer(r, g, view, proj, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        r.render(g, view, proj)
+    torch.cuda.synchronize()
+    r.pass_times(mode=2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render(g, view, proj)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, r.pass_times(mode=1)
+
+
+def rel_l2(a, b):
+    import numpy as np
+    a = np.asarray(a, np.float64)[..., :3]
+    b = np.asarray(b, np.float64)[..., :3]
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def equal_error_ms(err, sweep):
+    """ms/frame of the plain path at error `err`: log-log interpolation
+    between the two sweep points that bracket it (None outside the sweep)."""
+    import math
+    pts = sorted(sweep, key=lambda p: -p[0])  # decreasing error
+    for (e0, m0), (e1, m1) in zip(pts, pts[1:]):
+        if e0 >= err >= e1 and e0 > e1:
+            t = (math.log(e0) - math.log(err)) / (math.log(e0) - math.log(e1))
+            return math.exp(math.log(m0) + t * (math.log(m1) - math.log(m0)))
+    if pts and abs(pts[-1][0] - err) < 1e-12:
+        return pts[-1][1]
+    return None
+
+
+def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
+    """Denoised 4-direction path vs the plain path at equal error against the
+    FP64 256-direction gather."""
+    import numpy as np
+    from legitengine_amd import device as dev
+    from legitengine_amd import lgx
+    from legitengine_amd import denoiser as dn
+    path = args.weights or dn.WEIGHTS
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"trained U-Net weights not found at {path}")
+    net = dev.UNet(1, H, W)
+    net.load(path)
+    rden = dev.Renderer(W, H, DEN_DIRS, STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_UNET, kind="mixed",
+                        profile=True)
+    rden.attach_unet(net)
+    den_ms, den_passes = time_renderer(rden, g, view, proj, args.steps, args.warmup)
+    unet_ms = sum(p["ms"] for p in den_passes if p["name"] == "DenoiserPass")
+    flops = dn.unet_flops_per_pixel() * W * H
+    plain = {}
+    for d in PLAIN_SWEEP:
+        for kind, den in (("plain", lgx.DENOISER_BILATERAL), ("raw", lgx.DENOISER_NONE)):
+            r = dev.Renderer(W, H, d, STEPS, LEVELS, sampler, GRID, denoiser=den, denoise_passes=BILAT_PASSES,
+                             denoise_radius=BILAT_RADIUS, kind="mixed", profile=True)
+            ms, _ = time_renderer(r, g, view, proj, max(3, args.steps // 4), 1)
+            plain[f"{kind}{d}"] = {"renderer": r, "ms": ms, "err": []}
+    den_err, noisy_err = [], []
+    gs = dev.GBufferDev.alloc(W, H, "mixed")
+    target = dev.Gather(W, H, TARGET_DIRS, 0, sampler, precision=lgx.PREC_F64)
+    for s in seeds:
+        sc = lgx.make_scene(s, 32, 8)
+        v, p = lgx.scene_matrices(sc, W, H)
+        gs.render(sc)
+        pyr = dev.PyramidDev(gs, LEVELS, sampler).build()
+        tgt = target(v, p, gs, pyr).numpy()
+        rden.render(gs, v, p)
+        den_err.append(rel_l2(rden.output(), tgt))
+        noisy_err.append(rel_l2(rden.intermediate("indirect"), tgt))
+        for e in plain.values():
+            e["renderer"].render(gs, v, p)
+            e["err"].append(rel_l2(e["renderer"].output(), tgt))
+    err = float(np.mean(den_err))
+    sweep = {k: {"ms_per_frame": round(e["ms"], 4), "rel_l2": round(float(np.mean(e["err"])), 5)}
+             for k, e in plain.items()}
+    # the faster of the blurred and the raw plain path at each error level
+    pts = [(float(np.mean(e["err"])), e["ms"]) for e in plain.values()]
+    frontier = [pt for pt in pts if not any(q[0] <= pt[0] and q[1] < pt[1] for q in pts if q is not pt)]
+    eq_ms = equal_error_ms(err, frontier)
+    for e in plain.values():
+        e["renderer"].close()
+    return {
+        "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather, {STEPS} steps, {LEVELS}-level Hi-Z, 4x4 "
+                    "interleave, BF16 U-Net denoiser pass (3 levels, 32/64/128 channels, FP32 accumulation)",
+        "ms_per_frame": round(den_ms, 4),
+        "unet_ms": round(unet_ms, 4),
+        "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(den_passes)},
+        "rel_l2_vs_fp64_256": round(err, 5),
+        "rel_l2_per_seed": [round(x, 5) for x in den_err],
+        "noisy_rel_l2": round(float(np.mean(noisy_err)), 5),
+        "target": f"{TARGET_DIRS}-direction gather, reference step rule (no cap), FP64 on the GPU",
+        "seeds": seeds,
+        "plain_sweep": sweep,
+        "plain_ms_at_equal_error": round(eq_ms, 4) if eq_ms else None,
+        "speedup_at_equal_error": round(eq_ms / den_ms, 3) if eq_ms else None,
+        "speedup_vs_plain32": round(plain32_ms / den_ms, 3),
+        "unet_roofline": {"bound": "mfma", "achieved": round(flops / (unet_ms * 1e-3) / 1e12, 2),
+                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(flops / (unet_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
+                          "flops_per_frame": flops},
+        "weights": os.path.relpath(path, ROOT),
+    }
+
+
+def batch_resolution(seed):
+    """The view's resolution, drawn uniformly from the four sizes by its seed."""
+    import numpy as np
+    return BATCH_SIZES[int(np.random.default_rng(seed).integers(len(BATCH_SIZES)))]
+
+
+def batch_leg(args, sampler):
+    """32 views of uneven resolution, each its own seeded scene: one captured
+    hipGraph per batch (pyramids, ONE persistent gather launch over all views,
+    the U-Net per view), launch shape from the autotuner."""
+    import numpy as np
+    import torch
+    from legitengine_amd import device as dev
+    from legitengine_amd import lgx
+    from legitengine_amd import denoiser as dn
+    views, sizes = [], []
+    for s in BATCH_SEEDS:
+        w, h = batch_resolution(s)
+        sc = lgx.make_scene(s, 32, 8)
+        view, proj = lgx.scene_matrices(sc, w, h)
+        views.append((dev.GBufferDev.alloc(w, h, "mixed").render(sc), view, proj))
+        sizes.append((w, h))
+    maxw, maxh = max(w for w, _ in sizes), max(h for _, h in sizes)
+    net = dev.UNet(1, maxh, maxw)
+    path = args.weights or dn.WEIGHTS
+    net.load(path)
+    b = dev.Batch(views, dirs=DEN_DIRS, max_steps=STEPS, levels=LEVELS, sampler=sampler, grid=GRID,
+                  footprint_dirs=dn.NOISY_FOOTPRINT, net=net)
+    tune = b.autotune()
+    stream = torch.cuda.Stream()
+    b.capture(stream)
+    for _ in range(max(1, args.warmup)):
+        b.replay(stream)
+    stream.synchronize()
+    # throughput: back-to-back replays
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.replay(stream)
+    stream.synchronize()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # latency: per view, from the batch's start stamp to the view's stamp
+    lat, stamped = [], []
+    for _ in range(args.steps):
+        b.replay(stream)
+        one = b.latencies()  # synchronises; the stamps come from inside the replay
+        lat.extend(one)
+        stamped.append(max(one))
+    pixels = sum(w * h for w, h in sizes)
+    counts = {f"{w}x{h}": sizes.count((w, h)) for (w, h) in BATCH_SIZES}
+    tile, wg = b.shape()
+    b.close()
+    return {
+        "workload": f"{len(views)} views (seeds {BATCH_SEEDS[0]}-{BATCH_SEEDS[-1]}, own scenes), resolutions drawn "
+                    "from {512x512, 1280x720, 1920x1080, 2560x1440}: mips, Hi-Z and pre-blur per view, one "
+                    f"persistent {DEN_DIRS}-direction gather launch over all views, BF16 U-Net per view (smallest "
+                    "first); the batch is one hipGraph replay",
+        "views": counts,
+        "megapixels": round(pixels / 1e6, 3),
+        "batch_ms": round(batch_ms, 4),
+        "mpixel_per_s": round(pixels / 1e6 / (batch_ms * 1e-3), 1),
+        "stamped_batch_ms": round(float(np.median(stamped)), 4),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                       "min": round(float(min(lat)), 4)},
+        "autotune": {"best": tune["best"], "cus": tune["cus"],
+                     "table": [(r["tile"], r["workgroups"], round(r["ms"], 4)) for r in tune["table"]]},
+        "shape": {"tile": tile, "workgroups": wg},
+        "replays": args.steps,
+    }
 
 
 def cpu_baseline(g, view, proj, sampler, gpu_gather, sample_px):

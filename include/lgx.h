@@ -112,6 +112,9 @@ typedef struct {
   int32_t *step_counts; /* optional device int32[W*H]: on-screen steps per pixel (sum over dirs), NULL = off */
   int32_t tile;      /* launch shape: an index below lgx_gather_tiles() (0 = default) */
   int32_t roi[4];    /* x0, y0, w, h: gather only these pixels (w = 0: the whole frame); output stays full-frame */
+  int32_t footprint_dirs; /* direction count in the step growth and lod (indirectLighting.frag:212,217,234);
+                             <= 0 = dirs, as in the shader.  A few directions with a finer footprint feed the
+                             learned denoiser. */
 } lgx_gather_params;
 
 /* Workspace for the gather: per-(pattern, direction) screen directions and
@@ -218,7 +221,7 @@ typedef struct {
   int32_t composite; /* 1: add the final gathering pass */
   int32_t profile;   /* 1: per-pass hipEvent timings */
   int32_t validate;  /* 1: passes may only touch the views and buffers they declare */
-  int32_t reserved;
+  int32_t footprint_dirs; /* gather step growth / lod direction count (lgx_gather_params), <= 0 = dirs */
 } lgx_renderer_config;
 
 int lgx_renderer_create(const lgx_renderer_config *cfg, lgx_renderer **out);
@@ -288,6 +291,64 @@ int lgx_copy_image(const lgx_image *src, lgx_image *dst, lgx_stream_t stream);
 
 /* Number of gather launch shapes (lgx_gather_params.tile indexes them). */
 int lgx_gather_tiles(void);
+
+/* ---- several views of uneven resolution in one persistent launch ----
+ * No counterpart in the reference (its frame loop, main.cpp:269-312, renders
+ * one view per frame); built for serving many views per submission.
+ *
+ * lgx_gather_batch: the IndirectLightPass of n views as ONE launch of a
+ * persistent kernel whose workgroups walk the views' tiles.  Each item is
+ * what lgx_gather takes, checked the same way; all share LGX_PREC_F32, the
+ * sampler, the tile and the radiance / normal formats (RGBA16F + RG16F or
+ * RGBA32F + RG32F); step_counts must be NULL.  The descriptors and cameras
+ * are uploaded at creation (synchronous); the images must stay allocated. */
+typedef struct lgx_gather_batch lgx_gather_batch;
+int lgx_gather_batch_create(const lgx_gather_params *p, const void *const *workspaces, const lgx_gbuffer *g,
+                            const lgx_pyramid *pyr, lgx_image *out, int32_t n, lgx_gather_batch **res);
+/* Launch shape for every view (synchronous upload; not during a capture). */
+int lgx_gather_batch_set_tile(lgx_gather_batch *b, int32_t tile);
+/* Logical blocks (tiles x slices) of all views together. */
+int64_t lgx_gather_batch_blocks(const lgx_gather_batch *b);
+/* workgroups: persistent workgroups (rounded up to a multiple of 8);
+ * 0 = one per logical block. */
+int lgx_gather_batch_launch(const lgx_gather_batch *b, int32_t workgroups, lgx_stream_t stream);
+void lgx_gather_batch_destroy(lgx_gather_batch *b);
+
+/* lgx_batch: the denoised path for n views (own G-buffers and cameras):
+ * per view the mips, Hi-Z and pre-blur, then the 4-direction gather of all
+ * views in one lgx_gather_batch launch, then per view (smallest first) the
+ * U-Net denoiser, and a device timestamp as each view completes.  The whole
+ * batch can be captured into ONE hipGraph and replayed. */
+typedef struct lgx_batch lgx_batch;
+typedef struct {
+  int32_t width, height;
+  lgx_gbuffer gbuffer; /* caller's device images, width x height */
+  float view[16], proj[16];
+} lgx_batch_view;
+typedef struct {
+  int32_t dirs, max_steps, levels, grid, sampler, footprint_dirs;
+  int32_t tile;       /* gather launch shape */
+  int32_t workgroups; /* persistent gather workgroups, 0 = one per block */
+} lgx_batch_config;
+/* net: created with max n >= 1 and max h x w covering every view; NULL =
+ * no denoiser (the views' outputs are the noisy gathers). */
+int lgx_batch_create(const lgx_batch_config *c, const lgx_batch_view *views, int32_t n, struct lgx_unet *net,
+                     lgx_batch **out);
+/* Times every (tile, workgroups) pair on the batch's own gather launch,
+ * keeps the fastest and writes a JSON table to buf. */
+int lgx_batch_autotune(lgx_batch *b, lgx_stream_t stream, char *buf, size_t cap);
+int lgx_batch_run(lgx_batch *b, lgx_stream_t stream);
+int lgx_batch_capture(lgx_batch *b, lgx_stream_t stream); /* a non-default stream */
+int lgx_batch_replay(lgx_batch *b, lgx_stream_t stream);
+/* Per view: ms from the batch's start to that view's completion in the last
+ * run or replay (synchronises the device). */
+int lgx_batch_latencies(lgx_batch *b, double *ms, int32_t n);
+/* which: 0 = noisy gather, 1 = denoised (RGBA32F, width x height). */
+int lgx_batch_output(lgx_batch *b, int32_t i, int32_t which, lgx_image *out);
+/* The launch shape in use: tile and workgroups. */
+int lgx_batch_shape(const lgx_batch *b, int32_t *tile, int32_t *workgroups);
+const char *lgx_batch_last_error(const lgx_batch *b);
+void lgx_batch_destroy(lgx_batch *b);
 
 const char *lgx_strerror(int code);
 int32_t lgx_version(void);

@@ -186,6 +186,8 @@ class RenderGraph {
   void Capture(lgx_stream_t stream);
   void Replay(lgx_stream_t stream);
   bool HasCapture() const;
+  // Drops passes declared but not yet executed (a frame whose declaration failed).
+  void DiscardPasses() { passes_.clear(); }
 
   void SetProfiling(bool on) { profiling_ = on; }
   // Validation: record funcs may only touch views/buffers their pass declared.

@@ -156,9 +156,9 @@ template <class T> LGX_D size_t point_index(T sx, T sy, int lw, int lh, int pitc
 
 // T: per-step arithmetic; A: accumulation of the slice contributions and of
 // the direction sum (float/float, double/double, or float/double).
+// One workgroup's share: logical block b (slice k = b % grid^2, tile b / grid^2).
 template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT>
-__global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
-  const uint32_t b = xcd_remap(blockIdx.x, (uint32_t)a.n_blocks);
+LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
   const int g2 = a.grid * a.grid;
   const int k = (int)(b % (uint32_t)g2);
   const int t = (int)(b / (uint32_t)g2);
@@ -370,6 +370,33 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
   if (COUNT) a.step_counts[(size_t)y * a.w + x] = step_total;
 }
 
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT>
+__global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
+  gather_block<T, A, SAMPLER, RFMT, NFMT, COUNT>(a, xcd_remap(blockIdx.x, (uint32_t)a.n_blocks));
+}
+
+// Persistent gather over several views: the logical blocks of all views form
+// one list (view-major, first[v] = index of view v's first block, first[V] =
+// total).  The list is cut into 8 contiguous chunks, one per XCD (workgroup
+// w runs on XCD w % 8, gridDim.x is a multiple of 8), and each XCD's
+// workgroups stride through their chunk, so neighbouring tiles and the 16
+// slices of a tile share one L2.  Every workgroup ends when its stride
+// passes the chunk end.
+template <class T, class A, int SAMPLER, int RFMT, int NFMT>
+__global__ __launch_bounds__(256) void gather_multi_kernel(const GatherArgs<T> *__restrict__ views,
+                                                           const uint32_t *__restrict__ first, int n_views) {
+  const uint32_t X = 8;
+  const uint32_t total = first[n_views];
+  const uint32_t xcd = blockIdx.x % X, lane_wg = blockIdx.x / X, per_xcd = gridDim.x / X;
+  const uint32_t chunk = (total + X - 1) / X;
+  const uint32_t c0 = xcd * chunk, c1 = min(total, c0 + chunk);
+  int v = 0;
+  for (uint32_t it = c0 + lane_wg; it < c1; it += per_xcd) {
+    while (v + 1 < n_views && it >= first[v + 1]) v++;  // it only grows
+    gather_block<T, A, SAMPLER, RFMT, NFMT, false>(views[v], it - first[v]);
+  }
+}
+
 // ---------------------------------------------------------------- host side
 
 struct GatherTableShape {
@@ -387,11 +414,14 @@ static void hammersley(int i, int n, float &aa, float &bb) {
   bb = (float)v / (float)0xffffffffu;
 }
 
+// direction count that sets the step growth and the lod
+static int footprint(const lgx_gather_params *p) { return p->footprint_dirs > 0 ? p->footprint_dirs : p->dirs; }
+
 template <class T> static int table_steps(const lgx_gather_params *p) {
   const T pi = T(3.1415f);
   const T W = T(p->width), H = T(p->height);
   const T near_step = W / T(1000);
-  const T ratio = T(2) * pi / T(p->dirs) + T(1);
+  const T ratio = T(2) * pi / T(footprint(p)) + T(1);
   const T diag = std::sqrt(W * W + H * H);
   long it = (long)(std::log(diag / near_step) / std::log(ratio)) + 2;
   if (p->max_steps > 0 && it > p->max_steps) it = p->max_steps;
@@ -400,7 +430,8 @@ template <class T> static int table_steps(const lgx_gather_params *p) {
 }
 
 static bool gather_params_ok(const lgx_gather_params *p) {
-  return p && p->width > 0 && p->height > 0 && p->dirs >= 1 && p->dirs <= LGX_MAX_DIRS && p->grid >= 1 &&
+  return p && p->width > 0 && p->height > 0 && p->dirs >= 1 && p->dirs <= LGX_MAX_DIRS &&
+         p->footprint_dirs <= LGX_MAX_DIRS && p->grid >= 1 &&
          p->grid <= 8 && (p->sampler == LGX_SAMPLER_TRILINEAR || p->sampler == LGX_SAMPLER_HIZ_POINT) &&
          (p->precision == LGX_PREC_F32 || p->precision == LGX_PREC_F64 || p->precision == LGX_PREC_F32_ACC64) &&
          (p->out_layout == LGX_OUT_INTERLEAVED || p->out_layout == LGX_OUT_SLICES);
@@ -441,12 +472,13 @@ static bool fill_tables(const lgx_gather_params *p, const GatherTableShape &s, s
       dirs[((size_t)k * D + d) * 2 + 0] = std::cos(screen_ang);
       dirs[((size_t)k * D + d) * 2 + 1] = std::sin(screen_ang);
     }
-    const T ratio = T(2) * pi / T(D) + T(1);
+    const T Df = T(footprint(p));
+    const T ratio = T(2) * pi / Df + T(1);
     int prev_level = 0;
     for (int l = 0; l <= LGX_MAX_LEVELS; l++) lstart[k * (LGX_MAX_LEVELS + 1) + l] = l == 0 ? 0 : s.S_tab;
     for (int j = 0; j < s.S_tab; j++) {
       T off = near_step * std::pow(ratio, T(j) + lin_offset) + T(1) - near_step;
-      T lod = std::log(std::max(T(0), T(2) * pi / T(D) * (off - T(1)) * T(0.5f))) / std::log(T(2)) + T(-2);
+      T lod = std::log(std::max(T(0), T(2) * pi / Df * (off - T(1)) * T(0.5f))) / std::log(T(2)) + T(-2);
       steps[((size_t)k * s.S_tab + j) * 2 + 0] = off;
       steps[((size_t)k * s.S_tab + j) * 2 + 1] = lod;
       int level = lod > T(0) ? (int)std::min(std::floor(lod), T(LGX_MAX_LEVELS)) : 0;
@@ -546,18 +578,17 @@ extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
 
 extern "C" int lgx_gather_tiles(void) { return kNumTiles; }
 
-template <class T, class A>
-static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
-                       lgx_image *out, hipStream_t stream) {
+template <class T>
+static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+                     lgx_image *out, GatherArgs<T> &a) {
   GatherTableShape s = table_shape(p);
-  GatherArgs<T> a;
   memset(&a, 0, sizeof(a));
   frame_constants<T>(p, a);
   const T pi = T(3.1415f);
   a.W = T(p->width);
   a.H = T(p->height);
   a.near_step = a.W / T(1000);
-  a.log_ratio = std::log(T(2) * pi / T(p->dirs) + T(1));
+  a.log_ratio = std::log(T(2) * pi / T(footprint(p)) + T(1));
   a.ambient = T(0.01f);
   a.w = p->width;
   a.h = p->height;
@@ -609,6 +640,15 @@ static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbu
   a.tiles_x = (a.lcells_x + a.tile_w - 1) / a.tile_w;
   int tiles_y = (a.lcells_y + a.tile_h - 1) / a.tile_h;
   a.n_blocks = a.tiles_x * tiles_y * s.g2;
+  return LGX_OK;
+}
+
+template <class T, class A>
+static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+                       lgx_image *out, hipStream_t stream) {
+  GatherArgs<T> a;
+  int rc = make_args<T>(p, ws, g, pyr, out, a);
+  if (rc != LGX_OK) return rc;
   hipError_t e;
   const int rf = pyr->radiance[0].format, nf = g->normal.format;
   if (p->sampler == LGX_SAMPLER_HIZ_POINT)
@@ -618,8 +658,9 @@ static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbu
   return hip_status(e);
 }
 
-extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
-                          const lgx_pyramid *pyr, lgx_image *out, lgx_stream_t stream) {
+// The argument checks of lgx_gather.
+static int check_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
+                        const lgx_pyramid *pyr, const lgx_image *out) {
   if (!gather_params_ok(p) || !g || !pyr || !out) return LGX_EINVAL;
   if (!workspace) return LGX_EWORKSPACE;
   if (!image_ok(&g->depth) || !image_ok(&g->normal) || !image_ok(out)) return LGX_EINVAL;
@@ -645,8 +686,133 @@ extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, con
     ew /= 2;
     eh /= 2;
   }
+  return LGX_OK;
+}
+
+extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
+                          const lgx_pyramid *pyr, lgx_image *out, lgx_stream_t stream) {
+  const int rc = check_gather(p, workspace, g, pyr, out);
+  if (rc != LGX_OK) return rc;
   if (p->precision == LGX_PREC_F64) return gather_impl<double, double>(p, workspace, g, pyr, out, (hipStream_t)stream);
   if (p->precision == LGX_PREC_F32_ACC64)
     return gather_impl<float, double>(p, workspace, g, pyr, out, (hipStream_t)stream);
   return gather_impl<float, float>(p, workspace, g, pyr, out, (hipStream_t)stream);
 }
+
+// ---------------------------------------------------------------- several views, one persistent launch
+
+struct lgx_gather_batch {
+  std::vector<lgx_gather_params> p;
+  std::vector<const void *> ws;
+  std::vector<lgx_gbuffer> g;
+  std::vector<lgx_pyramid> pyr;
+  std::vector<lgx_image> out;
+  int sampler = 0, rf = 0, nf = 0, tile = 0;
+  uint32_t total = 0;
+  void *dev = nullptr;  // GatherArgs<float>[n], then uint32 first[n + 1]
+  ~lgx_gather_batch() {
+    if (dev) (void)hipFree(dev);
+  }
+};
+
+namespace lgx {
+
+static int gather_batch_upload(lgx_gather_batch *b, int tile) {
+  const int n = (int)b->p.size();
+  std::vector<GatherArgs<float>> args((size_t)n);
+  std::vector<uint32_t> first((size_t)n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    lgx_gather_params q = b->p[i];
+    q.tile = tile;
+    int rc = make_args<float>(&q, b->ws[i], &b->g[i], &b->pyr[i], &b->out[i], args[i]);
+    if (rc != LGX_OK) return rc;
+    first[i + 1] = first[i] + (uint32_t)args[i].n_blocks;
+  }
+  // no launch of this batch may still read the previous arguments
+  int rc0 = hip_status(hipDeviceSynchronize());
+  if (rc0 != LGX_OK) return rc0;
+  const size_t ab = args.size() * sizeof(GatherArgs<float>);
+  std::vector<unsigned char> blob(ab + first.size() * sizeof(uint32_t));
+  memcpy(blob.data(), args.data(), ab);
+  memcpy(blob.data() + ab, first.data(), first.size() * sizeof(uint32_t));
+  int rc = hip_status(hipMemcpy(b->dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  if (rc != LGX_OK) return rc;
+  b->tile = tile;
+  b->total = first[n];
+  return LGX_OK;
+}
+
+template <int SAMPLER, int RFMT, int NFMT>
+static hipError_t launch_multi(const lgx_gather_batch *b, unsigned grid, hipStream_t s) {
+  const int n = (int)b->p.size();
+  const GatherArgs<float> *args = (const GatherArgs<float> *)b->dev;
+  const uint32_t *first = (const uint32_t *)((const unsigned char *)b->dev + (size_t)n * sizeof(GatherArgs<float>));
+  hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT>), dim3(grid), dim3(256), 0, s, args,
+                     first, n);
+  return hipGetLastError();
+}
+
+}  // namespace lgx
+
+extern "C" int lgx_gather_batch_create(const lgx_gather_params *p, const void *const *workspaces,
+                                       const lgx_gbuffer *g, const lgx_pyramid *pyr, lgx_image *out, int32_t n,
+                                       lgx_gather_batch **res) {
+  if (!p || !workspaces || !g || !pyr || !out || !res || n < 1) return LGX_EINVAL;
+  *res = nullptr;
+  for (int i = 0; i < n; i++) {
+    int rc = check_gather(&p[i], workspaces[i], &g[i], &pyr[i], &out[i]);
+    if (rc != LGX_OK) return rc;
+    if (p[i].precision != LGX_PREC_F32 || p[i].step_counts) return LGX_EUNSUPPORTED;
+    if (p[i].sampler != p[0].sampler || p[i].tile != p[0].tile ||
+        pyr[i].radiance[0].format != pyr[0].radiance[0].format || g[i].normal.format != g[0].normal.format)
+      return LGX_EINVAL;
+  }
+  const int rf = pyr[0].radiance[0].format, nf = g[0].normal.format;
+  if (!((rf == LGX_RGBA16F && nf == LGX_RG16F) || (rf == LGX_RGBA32F && nf == LGX_RG32F))) return LGX_EFORMAT;
+  if (p[0].tile < 0 || p[0].tile >= kNumTiles) return LGX_EUNSUPPORTED;
+  lgx_gather_batch *b = new (std::nothrow) lgx_gather_batch;
+  if (!b) return LGX_EINVAL;
+  b->p.assign(p, p + n);
+  b->ws.assign(workspaces, workspaces + n);
+  b->g.assign(g, g + n);
+  b->pyr.assign(pyr, pyr + n);
+  b->out.assign(out, out + n);
+  b->sampler = p[0].sampler;
+  b->rf = rf;
+  b->nf = nf;
+  int rc = hip_status(hipMalloc(&b->dev, (size_t)n * sizeof(GatherArgs<float>) + ((size_t)n + 1) * sizeof(uint32_t)));
+  if (rc == LGX_OK) rc = gather_batch_upload(b, p[0].tile);
+  if (rc != LGX_OK) {
+    delete b;
+    return rc;
+  }
+  *res = b;
+  return LGX_OK;
+}
+
+extern "C" int lgx_gather_batch_set_tile(lgx_gather_batch *b, int32_t tile) {
+  if (!b) return LGX_EINVAL;
+  if (tile < 0 || tile >= kNumTiles) return LGX_EUNSUPPORTED;
+  if (tile == b->tile) return LGX_OK;
+  return gather_batch_upload(b, tile);
+}
+
+extern "C" int64_t lgx_gather_batch_blocks(const lgx_gather_batch *b) { return b ? (int64_t)b->total : -1; }
+
+extern "C" int lgx_gather_batch_launch(const lgx_gather_batch *b, int32_t workgroups, lgx_stream_t stream) {
+  if (!b || workgroups < 0) return LGX_EINVAL;
+  uint32_t grid = workgroups > 0 ? (uint32_t)workgroups : b->total;
+  grid = (grid + 7) / 8 * 8;  // the XCD chunking needs a multiple of 8
+  if (grid == 0 || grid > (1u << 24)) return LGX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (b->sampler == LGX_SAMPLER_HIZ_POINT)
+    e = b->rf == LGX_RGBA16F ? launch_multi<LGX_SAMPLER_HIZ_POINT, LGX_RGBA16F, LGX_RG16F>(b, grid, s)
+                             : launch_multi<LGX_SAMPLER_HIZ_POINT, LGX_RGBA32F, LGX_RG32F>(b, grid, s);
+  else
+    e = b->rf == LGX_RGBA16F ? launch_multi<LGX_SAMPLER_TRILINEAR, LGX_RGBA16F, LGX_RG16F>(b, grid, s)
+                             : launch_multi<LGX_SAMPLER_TRILINEAR, LGX_RGBA32F, LGX_RG32F>(b, grid, s);
+  return hip_status(e);
+}
+
+extern "C" void lgx_gather_batch_destroy(lgx_gather_batch *b) { delete b; }

@@ -151,6 +151,7 @@ struct SsvgiRenderer::ViewportResources {
     p.precision = LGX_PREC_F32;
     p.out_layout = LGX_OUT_SLICES;
     p.tile = c.tile;
+    p.footprint_dirs = c.footprint_dirs;
     return p;
   }
   MippedProxy depth, radiance, blurredRadiance;
@@ -363,6 +364,7 @@ extern "C" int lgx_renderer_create(const lgx_renderer_config *c, lgx_renderer **
     s.height = c->height;
     s.dirs = c->dirs;
     s.max_steps = c->max_steps;
+    s.footprint_dirs = c->footprint_dirs;
     s.levels = c->levels;
     s.grid = c->grid;
     s.sampler = c->sampler;
@@ -394,7 +396,12 @@ static void declare(lgx_renderer *r, const lgx_gbuffer *g, const float *view, co
   memcpy(f.view, view, 64);
   memcpy(f.proj, proj, 64);
   f.gbuffer = *g;
-  r->r->RenderFrame(f);
+  try {
+    r->r->RenderFrame(f);
+  } catch (...) {
+    r->rg.DiscardPasses();
+    throw;
+  }
 }
 
 extern "C" int lgx_renderer_render(lgx_renderer *r, const lgx_gbuffer *g, const float view[16], const float proj[16],

@@ -6,6 +6,9 @@ Pairs: a seeded scene (32 boxes, 8 spheres) seen from a seeded camera pose
 at 1920x1080; a 256x256 crop (origin on the 4-pixel interleave grid, so the
 pattern phase matches full frames) of the 4-direction gather is the input,
 the 256-direction gather accumulated in FP64 on the same crop the target.
+The input gather may march with the step growth and mip lod of a larger
+direction count (footprint_dirs), trading a few more steps for an input
+that is less blurred by coarse mips.
 Run: python -m legitengine_amd.denoiser train --scenes N --steps K --out PATH
 """
 import argparse
@@ -24,7 +27,18 @@ W, H = 1920, 1080
 LEVELS = 6
 CROP = 256
 NOISY_DIRS, TARGET_DIRS, PLAIN_DIRS, STEPS = 4, 256, 32, 64
+NOISY_FOOTPRINT = 0
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_1080p.safetensors")
+# (logical cin, cout, resolution divisor) of the ten 3x3 convolutions; the
+# 1x1 head maps 32 channels to 3
+UNET_CONVS = ((9, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (64, 128, 4), (128, 128, 4),
+              (192, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1))
+
+
+def unet_flops_per_pixel():
+    """Algorithmic forward FLOPs per full-resolution pixel (multiply-add = 2,
+    logical channel counts: the first layer's padding to 32 is not counted)."""
+    return sum(2 * 9 * ci * co / (d * d) for ci, co, d in UNET_CONVS) + 2 * 32 * 3
 
 
 class FrameGen:
@@ -38,10 +52,11 @@ class FrameGen:
         self.out32 = dev.DImage(w, h, lgx.RGBA32F)
         self.out64 = dev.DImage(w, h, lgx.RGBA64F)
 
-    def gather(self, dirs, steps=STEPS, precision=lgx.PREC_F32):
-        key = (dirs, steps, precision)
+    def gather(self, dirs, steps=STEPS, precision=lgx.PREC_F32, footprint=0):
+        key = (dirs, steps, precision, footprint)
         if key not in self.gathers:
-            self.gathers[key] = dev.Gather(self.w, self.h, dirs, steps, lgx.SAMPLER_HIZ_POINT, precision=precision)
+            self.gathers[key] = dev.Gather(self.w, self.h, dirs, steps, lgx.SAMPLER_HIZ_POINT, precision=precision,
+                                           footprint_dirs=footprint)
         return self.gathers[key]
 
     def view(self, scene_seed, pose=0, boxes=32, spheres=8):
@@ -66,13 +81,15 @@ class FrameGen:
         return rec
 
 
-def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx.PREC_F32_ACC64):
+def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx.PREC_F32_ACC64,
+                  footprints=(NOISY_FOOTPRINT,)):
+    """Inputs X[f] [n, CROP, CROP, 12] for each footprint f and targets T
+    [n, CROP, CROP, 4], all on the GPU."""
     fg = FrameGen()
     n = n_scenes * poses
-    X = torch.empty((n, CROP, CROP, 12), dtype=torch.float32, device="cuda")
+    X = {f: torch.empty((n, CROP, CROP, 12), dtype=torch.float32, device="cuda") for f in footprints}
     T = torch.empty((n, CROP, CROP, 4), dtype=torch.float32, device="cuda")
     rng = np.random.default_rng(seed0)
-    noisy_g = fg.gather(NOISY_DIRS)
     target_g = fg.gather(TARGET_DIRS, 0, target_precision)
     t0 = time.time()
     last = t0
@@ -82,9 +99,10 @@ def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx
         x0 = int(rng.integers(0, (W - CROP) // 4 + 1)) * 4
         y0 = int(rng.integers(0, (H - CROP) // 4 + 1)) * 4
         roi = (x0, y0, CROP, CROP)
-        noisy_g(view, proj, fg.g, fg.pyr, out=fg.out32, roi=roi)
+        for f in footprints:
+            fg.gather(NOISY_DIRS, footprint=f)(view, proj, fg.g, fg.pyr, out=fg.out32, roi=roi)
+            X[f][i] = fg.records(fg.out32, y0, x0, CROP, CROP)
         target_g(view, proj, fg.g, fg.pyr, out=fg.out64, roi=roi)
-        X[i] = fg.records(fg.out32, y0, x0, CROP, CROP)
         T[i] = fg.out64.t[y0:y0 + CROP, x0:x0 + CROP].float()
         if time.time() - last > 20 or i == n - 1:
             torch.cuda.synchronize()
@@ -93,7 +111,15 @@ def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx
     return X, T
 
 
-def train(net, X, T, steps, batch=16, lr=1e-3, log=print, seed=0, log_every=50):
+def lr_at(it, steps, lr, schedule):
+    """Learning rate of step `it`: constant, or a cosine decay from lr to lr/20."""
+    if schedule == "const":
+        return lr
+    import math
+    return lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * it / max(1, steps - 1))))
+
+
+def train(net, X, T, steps, batch=16, lr=1e-3, log=print, seed=0, log_every=50, schedule="cosine"):
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
@@ -105,7 +131,7 @@ def train(net, X, T, steps, batch=16, lr=1e-3, log=print, seed=0, log_every=50):
         xb = X.index_select(0, idx).contiguous()
         tb = T.index_select(0, idx).contiguous()
         loss.zero_()
-        net.train_step(xb, tb, lr, loss)
+        net.train_step(xb, tb, lr_at(it, steps, lr, schedule), loss)
         if it % log_every == 0 or it == steps - 1:
             hist.append((it, float(loss.item())))
             log(f"step {it} loss {hist[-1][1]:.5f} {time.time() - t0:.1f} s")
@@ -119,7 +145,7 @@ def rel_l2(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
-def evaluate(net, seeds, log=print, plain_dirs=(4, 8, 16, 32)):
+def evaluate(net, seeds, log=print, plain_dirs=(4, 8, 16, 32), footprint=NOISY_FOOTPRINT):
     """rel-L2 of the denoised 4-direction path and of plain paths against the
     256-direction FP64 gather on full 1080p frames."""
     fg = FrameGen()
@@ -129,12 +155,12 @@ def evaluate(net, seeds, log=print, plain_dirs=(4, 8, 16, 32)):
         fg.gather(TARGET_DIRS, 0, lgx.PREC_F64)(view, proj, fg.g, fg.pyr, out=fg.out64)
         tgt = fg.out64.t.clone()
         row = {"seed": s}
-        fg.gather(NOISY_DIRS)(view, proj, fg.g, fg.pyr, out=fg.out32)
+        fg.gather(NOISY_DIRS, footprint=footprint)(view, proj, fg.g, fg.pyr, out=fg.out32)
         row["noisy4"] = rel_l2(fg.out32.t, tgt)
         den = net.denoise(fg.g, fg.out32)
         row["denoised4"] = rel_l2(den.t, tgt)
         for d in plain_dirs:
-            o = fg.gather(d)(view, proj, fg.g, fg.pyr, out=fg.out32)
+            o = fg.gather(d, STEPS if d <= PLAIN_DIRS else 0)(view, proj, fg.g, fg.pyr, out=fg.out32)
             b = o
             for _ in range(2):
                 b = dev.bilateral(b, fg.g, radius=4)
@@ -159,9 +185,12 @@ def main(argv=None):
     t.add_argument("--resume", default=None)
     t.add_argument("--eval-seeds", default="200,201")
     t.add_argument("--report", default=None)
+    t.add_argument("--footprint", type=int, default=NOISY_FOOTPRINT)
+    t.add_argument("--schedule", default="cosine", choices=("cosine", "const"))
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", default="200,201,202,203,204,205,206,207")
+    e.add_argument("--footprint", type=int, default=NOISY_FOOTPRINT)
     args = ap.parse_args(argv)
     dev.require_device()
 
@@ -169,26 +198,28 @@ def main(argv=None):
         print(msg, flush=True)
 
     if args.cmd == "train":
-        X, T = build_dataset(args.scenes, args.poses, log=log)
+        X, T = build_dataset(args.scenes, args.poses, log=log, footprints=(args.footprint,))
+        X = X[args.footprint]
         net = dev.UNet(args.batch, CROP, CROP, seed=0)
         if args.resume:
             net.load(args.resume)
-        hist, secs = train(net, X, T, args.steps, args.batch, args.lr, log=log)
+        hist, secs = train(net, X, T, args.steps, args.batch, args.lr, log=log, schedule=args.schedule)
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
         net.save(args.out)
         log(f"saved {args.out}: {args.steps} steps in {secs:.1f} s ({args.steps / secs:.2f} steps/s)")
         del X, T
         infer = dev.UNet(1, H, W)
         infer.load(args.out)
-        rows = evaluate(infer, [int(s) for s in args.eval_seeds.split(",") if s])
+        rows = evaluate(infer, [int(s) for s in args.eval_seeds.split(",") if s], footprint=args.footprint)
         if args.report:
             with open(args.report, "w") as f:
                 json.dump({"history": hist, "train_seconds": secs, "steps": args.steps, "batch": args.batch,
-                           "scenes": args.scenes, "poses": args.poses, "eval": rows}, f)
+                           "scenes": args.scenes, "poses": args.poses, "footprint": args.footprint,
+                           "schedule": args.schedule, "eval": rows}, f)
     else:
         net = dev.UNet(1, H, W)
         net.load(args.weights)
-        evaluate(net, [int(s) for s in args.seeds.split(",") if s])
+        evaluate(net, [int(s) for s in args.seeds.split(",") if s], footprint=args.footprint)
     return 0
 
 

@@ -133,9 +133,10 @@ class Gather:
     """Prepared gather (workspace tables for one resolution / parameter set)."""
 
     def __init__(self, w, h, dirs, max_steps, sampler, grid=4, precision=lgx.PREC_F32, out_layout=lgx.OUT_INTERLEAVED,
-                 tile=0):
+                 tile=0, footprint_dirs=0):
         p = lgx.GatherParams()
         p.width, p.height, p.dirs, p.max_steps, p.grid = w, h, dirs, max_steps, grid
+        p.footprint_dirs = footprint_dirs
         p.sampler, p.precision, p.out_layout, p.tile = sampler, precision, out_layout, tile
         self.p = p
         nbytes = lgx.lib().lgx_gather_workspace_bytes(C.byref(p))
@@ -183,7 +184,7 @@ class Renderer:
 
     def __init__(self, w, h, dirs=32, max_steps=64, levels=6, sampler=lgx.SAMPLER_HIZ_POINT, grid=4,
                  denoiser=lgx.DENOISER_BILATERAL, denoise_passes=2, denoise_radius=4, kind="mixed", tile=0,
-                 composite=False, profile=False, validate=False):
+                 composite=False, profile=False, validate=False, footprint_dirs=0):
         f = lgx.GBUF_FORMATS[kind]
         c = lgx.RendererConfig()
         c.width, c.height, c.dirs, c.max_steps, c.levels, c.grid, c.sampler = w, h, dirs, max_steps, levels, grid, sampler
@@ -191,6 +192,7 @@ class Renderer:
         c.depth_sharpness, c.normal_sharpness = 10.5, 20.0
         c.normal_format, c.albedo_format, c.radiance_format = f["normal"], f["albedo"], f["radiance"]
         c.tile, c.composite, c.profile, c.validate = tile, int(composite), int(profile), int(validate)
+        c.footprint_dirs = footprint_dirs
         self.cfg = c
         h_ = C.c_void_p()
         lgx.check(lgx.lib().lgx_renderer_create(C.byref(c), C.byref(h_)), "lgx_renderer_create")
@@ -324,6 +326,125 @@ class UNet:
     def close(self):
         if self.h:
             lgx.lib().lgx_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GatherBatch:
+    """Several views' gathers in one persistent launch (lgx_gather_batch_*).
+    items: (Gather, GBufferDev, PyramidDev, out DImage, view, proj) tuples."""
+
+    def __init__(self, items):
+        n = len(items)
+        ps = (lgx.GatherParams * n)()
+        self._keep = items
+        for i, (gp, gb, pyr, out, view, proj) in enumerate(items):
+            C.memmove(C.byref(ps[i]), C.byref(gp.p), C.sizeof(lgx.GatherParams))
+            ps[i].view[:] = [float(v) for v in view]
+            ps[i].proj[:] = [float(v) for v in proj]
+            ps[i].roi[:] = [0, 0, 0, 0]
+            ps[i].step_counts = None
+        ws = (C.c_void_p * n)(*[it[0].ws.data_ptr() for it in items])
+        gs = (lgx.GBuffer * n)(*[it[1].desc for it in items])
+        pyrs = (lgx.Pyramid * n)(*[it[2].desc for it in items])
+        outs = (lgx.Image * n)(*[it[3].desc for it in items])
+        h = C.c_void_p()
+        lgx.check(lgx.lib().lgx_gather_batch_create(ps, ws, gs, pyrs, outs, n, C.byref(h)), "lgx_gather_batch_create")
+        self.h = h
+
+    def set_tile(self, tile):
+        lgx.check(lgx.lib().lgx_gather_batch_set_tile(self.h, tile), "lgx_gather_batch_set_tile")
+
+    @property
+    def blocks(self):
+        return int(lgx.lib().lgx_gather_batch_blocks(self.h))
+
+    def launch(self, workgroups=0, stream=None):
+        lgx.check(lgx.lib().lgx_gather_batch_launch(self.h, workgroups, stream_handle(stream)),
+                  "lgx_gather_batch_launch")
+
+    def close(self):
+        if self.h:
+            lgx.lib().lgx_gather_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """The denoised path for views of uneven resolution (lgx_batch_*).
+    views: (GBufferDev, view, proj) tuples; net: UNet or None."""
+
+    def __init__(self, views, dirs=4, max_steps=64, levels=6, sampler=lgx.SAMPLER_HIZ_POINT, grid=4,
+                 footprint_dirs=0, tile=0, workgroups=0, net=None):
+        n = len(views)
+        vs = (lgx.BatchView * n)()
+        for i, (gb, view, proj) in enumerate(views):
+            vs[i].width, vs[i].height = gb.w, gb.h
+            vs[i].gbuffer = gb.desc
+            vs[i].view[:] = [float(v) for v in view]
+            vs[i].proj[:] = [float(v) for v in proj]
+        c = lgx.BatchConfig()
+        c.dirs, c.max_steps, c.levels, c.grid, c.sampler = dirs, max_steps, levels, grid, sampler
+        c.footprint_dirs, c.tile, c.workgroups = footprint_dirs, tile, workgroups
+        self._keep = (views, net)
+        self.n = n
+        h = C.c_void_p()
+        lgx.check(lgx.lib().lgx_batch_create(C.byref(c), vs, n, net.h if net is not None else None, C.byref(h)),
+                  "lgx_batch_create")
+        self.h = h
+
+    def _err(self, code, what):
+        if code != lgx.OK:
+            raise lgx.LgxError(code, f"{what}: {lgx.lib().lgx_batch_last_error(self.h).decode()}")
+
+    def autotune(self, stream=None):
+        import json
+        buf = C.create_string_buffer(1 << 14)
+        self._err(lgx.lib().lgx_batch_autotune(self.h, stream_handle(stream), buf, len(buf)), "autotune")
+        return json.loads(buf.value.decode())
+
+    def run(self, stream=None):
+        self._err(lgx.lib().lgx_batch_run(self.h, stream_handle(stream)), "run")
+
+    def capture(self, stream):
+        self._err(lgx.lib().lgx_batch_capture(self.h, stream_handle(stream)), "capture")
+
+    def replay(self, stream=None):
+        self._err(lgx.lib().lgx_batch_replay(self.h, stream_handle(stream)), "replay")
+
+    def latencies(self):
+        a = (C.c_double * self.n)()
+        self._err(lgx.lib().lgx_batch_latencies(self.h, a, self.n), "latencies")
+        return list(a)
+
+    def shape(self):
+        t, w = C.c_int32(), C.c_int32()
+        self._err(lgx.lib().lgx_batch_shape(self.h, C.byref(t), C.byref(w)), "shape")
+        return t.value, w.value
+
+    def output(self, i, which=1):
+        """Copy of view i's noisy (0) or denoised (1) image as numpy (synchronises)."""
+        im = lgx.Image()
+        self._err(lgx.lib().lgx_batch_output(self.h, i, which, C.byref(im)), "output")
+        dst = DImage(im.w, im.h, im.format)
+        d = dst.desc
+        lgx.check(lgx.lib().lgx_copy_image(C.byref(im), C.byref(d), stream_handle()), "lgx_copy_image")
+        torch.cuda.synchronize()
+        return dst.numpy()
+
+    def close(self):
+        if self.h:
+            lgx.lib().lgx_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -39,7 +39,7 @@ class GatherParams(C.Structure):
     _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("width", C.c_int32), ("height", C.c_int32),
                 ("dirs", C.c_int32), ("max_steps", C.c_int32), ("grid", C.c_int32), ("sampler", C.c_int32),
                 ("precision", C.c_int32), ("out_layout", C.c_int32), ("step_counts", C.c_void_p),
-                ("tile", C.c_int32), ("roi", C.c_int32 * 4)]
+                ("tile", C.c_int32), ("roi", C.c_int32 * 4), ("footprint_dirs", C.c_int32)]
 
 
 class BilateralParams(C.Structure):
@@ -62,10 +62,21 @@ class RendererConfig(C.Structure):
                 ("denoise_passes", C.c_int32), ("denoise_radius", C.c_int32), ("depth_sharpness", C.c_float),
                 ("normal_sharpness", C.c_float), ("normal_format", C.c_int32), ("albedo_format", C.c_int32),
                 ("radiance_format", C.c_int32), ("tile", C.c_int32), ("composite", C.c_int32),
-                ("profile", C.c_int32), ("validate", C.c_int32), ("reserved", C.c_int32)]
+                ("profile", C.c_int32), ("validate", C.c_int32), ("footprint_dirs", C.c_int32)]
 
 
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_REGRESSION, DENOISER_UNET = 0, 1, 2, 3
+
+
+class BatchView(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("gbuffer", GBuffer), ("view", C.c_float * 16),
+                ("proj", C.c_float * 16)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("dirs", C.c_int32), ("max_steps", C.c_int32), ("levels", C.c_int32), ("grid", C.c_int32),
+                ("sampler", C.c_int32), ("footprint_dirs", C.c_int32), ("tile", C.c_int32),
+                ("workgroups", C.c_int32)]
 
 # Every entry point declared in include/lgx.h, with its ctypes signature.
 SIGNATURES = {
@@ -115,6 +126,23 @@ SIGNATURES = {
     "lgx_unet_denoise": (C.c_int, [C.c_void_p, C.POINTER(GBuffer), C.POINTER(Image), C.POINTER(Image), C.c_void_p]),
     "lgx_unet_mfma_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lgx_unet_destroy": (None, [C.c_void_p]),
+    "lgx_gather_batch_create": (C.c_int, [C.POINTER(GatherParams), C.POINTER(C.c_void_p), C.POINTER(GBuffer),
+                                          C.POINTER(Pyramid), C.POINTER(Image), C.c_int32, C.POINTER(C.c_void_p)]),
+    "lgx_gather_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lgx_gather_batch_blocks": (C.c_int64, [C.c_void_p]),
+    "lgx_gather_batch_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "lgx_gather_batch_destroy": (None, [C.c_void_p]),
+    "lgx_batch_create": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(BatchView), C.c_int32, C.c_void_p,
+                                   C.POINTER(C.c_void_p)]),
+    "lgx_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]),
+    "lgx_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lgx_batch_capture": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lgx_batch_replay": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lgx_batch_latencies": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32]),
+    "lgx_batch_output": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Image)]),
+    "lgx_batch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lgx_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "lgx_batch_destroy": (None, [C.c_void_p]),
     "lgx_strerror": (C.c_char_p, [C.c_int]),
     "lgx_version": (C.c_int32, []),
 }

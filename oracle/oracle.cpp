@@ -309,7 +309,10 @@ void gather_pixel(const orc_gather_params &p, const M4<T> &invVP, V3<T> cam, int
   const T near_step = W / T(1000);     /* :167 */
   const T ambient = T(0.01f);          /* :154 */
   const T pixel_ang_offset = T(2) * pi / T(D) * ang_offset; /* :170 */
-  const T ratio = T(2) * pi / T(D) + T(1);
+  /* the shader uses D for the step growth and the lod as well; a separate
+     footprint count lets a few directions march with a finer footprint */
+  const T Df = T(p.footprint_dirs > 0 ? p.footprint_dirs : D);
+  const T ratio = T(2) * pi / Df + T(1);
 
   T sum[3] = {0, 0, 0};
   int32_t steps_total = 0;
@@ -351,7 +354,7 @@ void gather_pixel(const orc_gather_params &p, const M4<T> &invVP, V3<T> cam, int
       }
       if (side > T(0)) steps_total++;
       const T blur_lod_offset = T(-2), lod_mult = T(0.5f); /* :231-233 */
-      T lod = std::log(std::max(T(0), T(2) * pi / T(D) * (off - T(1)) * lod_mult)) / std::log(T(2)) + blur_lod_offset;
+      T lod = std::log(std::max(T(0), T(2) * pi / Df * (off - T(1)) * lod_mult)) / std::log(T(2)) + blur_lod_offset;
       T ds[4];
       if (p.sampler == 0)
         sample_trilinear<T>(ld, p.n_levels, 1, sq.x, sq.y, lod, ds); /* :240 */

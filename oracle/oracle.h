@@ -33,7 +33,7 @@ typedef struct {
   int32_t grid;      /* interleave pattern size (reference: 4, :141) */
   int32_t sampler;   /* 0 = trilinear over avg+blur pyramids (reference), 1 = Hi-Z point fetch at floor(lod) */
   int32_t n_levels;  /* pyramid levels present */
-  int32_t pad;
+  int32_t footprint_dirs; /* direction count in the step growth (:212, :217) and lod (:234); <= 0 = dirs */
 } orc_gather_params;
 
 uint32_t orc_version(void);

@@ -90,6 +90,39 @@ def test_gather_workspace_and_argument_checks():
     assert L.lgx_gather(None, None, None, None, None, None) == lgx.EINVAL
 
 
+def test_batch_argument_checks():
+    """Host-side checks of the batch entry points: they fail before touching
+    a device."""
+    import ctypes as C
+    L = lgx.lib()
+    h = C.c_void_p()
+    assert L.lgx_gather_batch_create(None, None, None, None, None, 0, C.byref(h)) == lgx.EINVAL
+    assert L.lgx_batch_create(None, None, 0, None, C.byref(h)) == lgx.EINVAL
+    assert L.lgx_gather_batch_launch(None, 0, None) == lgx.EINVAL
+    assert L.lgx_batch_run(None, None) == lgx.EINVAL and L.lgx_batch_replay(None, None) == lgx.EINVAL
+    assert L.lgx_gather_batch_blocks(None) == -1
+    # descriptors that pass lgx_gather's checks (never dereferenced on the host)
+    w, hh = 64, 32
+    fake = 0x100000
+    p = lgx.GatherParams()
+    p.width, p.height, p.dirs, p.max_steps, p.grid, p.sampler = w, hh, 4, 16, 4, lgx.SAMPLER_HIZ_POINT
+    g = lgx.GBuffer()
+    g.depth = lgx.image(fake, w, hh, lgx.R32F)
+    g.normal = lgx.image(fake, w, hh, lgx.RG16F)
+    pyr = lgx.Pyramid()
+    pyr.n_levels = 1
+    pyr.depth[0] = lgx.image(fake, w, hh, lgx.R32F)
+    pyr.radiance[0] = lgx.image(fake, w, hh, lgx.RGBA16F)
+    out = lgx.image(fake, w, hh, lgx.RGBA32F)
+    ws = (C.c_void_p * 1)(fake)
+    for field, value, code in (("precision", lgx.PREC_F64, lgx.EFORMAT), ("tile", 99, lgx.EUNSUPPORTED)):
+        q = lgx.GatherParams()
+        C.memmove(C.byref(q), C.byref(p), C.sizeof(q))
+        setattr(q, field, value)
+        rc = L.lgx_gather_batch_create(C.byref(q), ws, C.byref(g), C.byref(pyr), C.byref(out), 1, C.byref(h))
+        assert rc == code and not h.value, (field, rc)
+
+
 @pytest.fixture(scope="module")
 def const_div_checker(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("cdiv") / "check_const_div")
@@ -98,7 +131,8 @@ def const_div_checker(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("extents", [("1920", "1080"), ("256", "320", "192", "96", "64", "48", "128", "640", "360")])
+@pytest.mark.parametrize("extents", [("1920", "1080"), ("256", "320", "192", "96", "64", "48", "128", "640", "360"),
+                                     ("512", "1280", "720", "2560", "1440")])
 def test_exact_extent_division(const_div_checker, extents):
     r = subprocess.run([const_div_checker, *extents], capture_output=True, text=True, timeout=300)
     assert r.stdout.strip() == "ok", r.stdout

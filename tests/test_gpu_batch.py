@@ -1,0 +1,132 @@
+"""Views of uneven resolution in one persistent launch (lgx_gather_batch_*)
+and the batched denoised path (lgx_batch_*), against the single-view entry
+points.
+
+Every logical block of the persistent kernel runs the same per-block code as
+lgx_gather, so outputs must be bit-identical for every launch shape; the
+batch's denoised images must equal lgx_unet_denoise on its own noisy
+images, eagerly and on every replay of the captured graph.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from legitengine_amd import lgx  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(256, 256), (320, 192), (192, 128)]
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from legitengine_amd import device
+    device.require_device()
+    return device
+
+
+@pytest.fixture(scope="module")
+def views(dev):
+    out = []
+    for i, (w, h) in enumerate(SIZES):
+        sc = lgx.make_scene(40 + i, 8, 2)
+        view, proj = lgx.scene_matrices(sc, w, h)
+        g = dev.GBufferDev.alloc(w, h, "mixed").render(sc)
+        out.append((g, view, proj))
+    torch.cuda.synchronize()
+    return out
+
+
+def single(dev, g, view, proj, tile=0, dirs=4, steps=64, footprint=0):
+    pyr = dev.PyramidDev(g, 6, lgx.SAMPLER_HIZ_POINT).build()
+    ga = dev.Gather(g.w, g.h, dirs, steps, lgx.SAMPLER_HIZ_POINT, tile=tile, footprint_dirs=footprint)
+    return ga(view, proj, g, pyr).numpy()
+
+
+@pytest.mark.parametrize("footprint", [0, 32])
+def test_gather_batch_matches_single_view(dev, views, footprint):
+    items, refs = [], []
+    for g, view, proj in views:
+        pyr = dev.PyramidDev(g, 6, lgx.SAMPLER_HIZ_POINT).build()
+        ga = dev.Gather(g.w, g.h, 4, 64, lgx.SAMPLER_HIZ_POINT, footprint_dirs=footprint)
+        items.append((ga, g, pyr, dev.DImage(g.w, g.h, lgx.RGBA32F), view, proj))
+    gb = dev.GatherBatch(items)
+    expect_blocks = sum(((w // 4 + 15) // 16) * ((h // 4 + 15) // 16) * 16 for (w, h) in SIZES)
+    assert gb.blocks == expect_blocks
+    for tile in range(lgx.lib().lgx_gather_tiles()):
+        gb.set_tile(tile)
+        refs = [single(dev, g, v, p, tile, footprint=footprint) for (g, v, p) in views]
+        for wg in (0, 8, 64, 1000):
+            for it in items:
+                it[3].t.fill_(-1.0)
+            gb.launch(wg)
+            torch.cuda.synchronize()
+            for it, ref in zip(items, refs):
+                assert it[3].numpy().tobytes() == ref.tobytes(), (tile, wg)
+    gb.close()
+
+
+def test_gather_batch_rejects_mixed_formats(dev, views):
+    g0, v0, p0 = views[0]
+    gf = dev.GBufferDev.from_host(lgx.render_gbuffer_cpu(lgx.make_scene(1, 8, 2), 64, 64, "f32"))
+    items = []
+    for g, view, proj in [(g0, v0, p0), (gf, v0, p0)]:
+        pyr = dev.PyramidDev(g, 4, lgx.SAMPLER_HIZ_POINT).build()
+        ga = dev.Gather(g.w, g.h, 4, 64, lgx.SAMPLER_HIZ_POINT)
+        items.append((ga, g, pyr, dev.DImage(g.w, g.h, lgx.RGBA32F), view, proj))
+    with pytest.raises(lgx.LgxError):
+        dev.GatherBatch(items)
+
+
+@pytest.fixture(scope="module")
+def net(dev):
+    return dev.UNet(1, 256, 320, seed=11)
+
+
+def test_batch_matches_pieces_and_orders_latencies(dev, views, net):
+    b = dev.Batch(views, dirs=4, max_steps=64, levels=6, net=net)
+    b.run()
+    torch.cuda.synchronize()
+    for i, (g, view, proj) in enumerate(views):
+        noisy = b.output(i, 0)
+        assert noisy.tobytes() == single(dev, g, view, proj).tobytes()
+        ref = net.denoise(g, dev.DImage.from_numpy(noisy, lgx.RGBA32F)).numpy()
+        assert b.output(i, 1).tobytes() == ref.tobytes()
+    lat = b.latencies()
+    assert all(x > 0 for x in lat)
+    # denoised smallest first: latency grows with the pixel count
+    order = np.argsort([w * h for (w, h) in SIZES], kind="stable")
+    assert all(lat[order[k]] <= lat[order[k + 1]] for k in range(len(order) - 1)), lat
+    b.close()
+
+
+def test_batch_capture_replay_and_autotune(dev, views, net):
+    b = dev.Batch(views, dirs=4, max_steps=64, levels=6, net=net)
+    rep = b.autotune()
+    assert len(rep["table"]) == 6 * lgx.lib().lgx_gather_tiles()
+    assert b.shape() == (rep["best"]["tile"], rep["best"]["workgroups"])
+    b.run()
+    torch.cuda.synchronize()
+    eager = [b.output(i, 1) for i in range(len(views))]
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.capture(s)
+    s.synchronize()
+    for r in range(3):
+        for i in range(len(views)):
+            lgx.check(lgx.lib().lgx_copy_image(C_image(b, i, 0), C_image(b, i, 1), dev.stream_handle()))
+        torch.cuda.synchronize()  # outputs overwritten with other data between replays
+        b.replay(s)
+        s.synchronize()
+        for i in range(len(views)):
+            assert b.output(i, 1).tobytes() == eager[i].tobytes(), (r, i)
+        assert all(x > 0 for x in b.latencies())
+    b.close()
+
+
+def C_image(b, i, which):
+    import ctypes as C
+    im = lgx.Image()
+    lgx.check(lgx.lib().lgx_batch_output(b.h, i, which, C.byref(im)))
+    return C.byref(im)

@@ -133,6 +133,25 @@ def test_gather_c1(dev, c1, seed_idx, sampler, dirs, steps):
     assert np.array_equal(counts, ref_steps)
 
 
+@pytest.mark.parametrize("footprint", [32, 64])
+def test_gather_footprint(dev, c1, footprint):
+    """4 directions marching with a 32- or 64-direction footprint (the
+    denoiser's input) against the oracle with the same footprint."""
+    sc, view, proj, gb = c1[1]
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, 5, lgx.SAMPLER_HIZ_POINT).build()
+    counts = torch.zeros((256, 256), dtype=torch.int32, device="cuda")
+    out = dev.Gather(256, 256, 4, 64, lgx.SAMPLER_HIZ_POINT, footprint_dirs=footprint)(view, proj, g, pyr,
+                                                                                       step_counts=counts)
+    torch.cuda.synchronize()
+    dep_b, rad_b, _, _ = _pyr_host(gb, 5, lgx.SAMPLER_HIZ_POINT)
+    ref, ref_steps = orc.gather(view, proj, gb["depth"], orc.widen(gb["normal"]), dep_b, rad_b, 4, 64,
+                                lgx.SAMPLER_HIZ_POINT, with_steps=True, footprint_dirs=footprint)
+    err = orc.rel_l2(out.numpy(), ref)
+    assert err <= GATHER_TOL, err
+    assert np.array_equal(counts.cpu().numpy(), ref_steps)
+
+
 @pytest.mark.parametrize("sampler", [lgx.SAMPLER_HIZ_POINT, lgx.SAMPLER_TRILINEAR])
 def test_gather_c2_crops(dev, c2, sampler):
     sc, view, proj, gb = c2[0]

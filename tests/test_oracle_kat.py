@@ -389,6 +389,37 @@ def test_gather_linear_in_radiance_and_float_vs_double():
     assert (s_cap <= 8 * 4).all() and (s_cap <= s_all).all()
 
 
+def test_gather_footprint_dirs():
+    """footprint_dirs replaces D in the step growth and the lod only: equal
+    to D it is the shader; larger it marches more, finer steps at finer mips
+    over the same D directions."""
+    from legitengine_amd import lgx
+    sc = lgx.make_scene(12, 8, 2)
+    w, h = 96, 64
+    view, proj = lgx.scene_matrices(sc, w, h)
+    gb = lgx.render_gbuffer_cpu(sc, w, h, "f32")
+    depth, octn = gb["depth"], orc.widen(gb["normal"])
+    rad = orc.widen(gb["radiance"])
+    dep_b, rad_b, _, _ = orc.build_pyramids(depth, rad, 4, 1, False)
+    a, sa = orc.gather(view, proj, depth, octn, dep_b, rad_b, 4, 0, 1, with_steps=True)
+    b, sb = orc.gather(view, proj, depth, octn, dep_b, rad_b, 4, 0, 1, with_steps=True, footprint_dirs=4)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    c, sc32 = orc.gather(view, proj, depth, octn, dep_b, rad_b, 4, 0, 1, with_steps=True, footprint_dirs=32)
+    # uncapped step count grows as 1 / log(2 pi / F + 1): about 5.3x from F = 4 to 32
+    ratio = sc32.sum() / sa.sum()
+    assert 4.0 < ratio < 6.0, ratio
+    assert not np.array_equal(a, c)
+    # a uniform radiance pyramid: every contribution is (radiance - ambient) h, so
+    # the result stays linear in it whatever the footprint
+    g1 = orc.gather(view, proj, depth, octn, dep_b, [np.ones_like(r) for r in rad_b], 4, 0, 1, precision="f64",
+                    footprint_dirs=32)
+    g2 = orc.gather(view, proj, depth, octn, dep_b, [2 * np.ones_like(r) for r in rad_b], 4, 0, 1, precision="f64",
+                    footprint_dirs=32)
+    g0 = orc.gather(view, proj, depth, octn, dep_b, [np.zeros_like(r) for r in rad_b], 4, 0, 1, precision="f64",
+                    footprint_dirs=32)
+    assert np.allclose(g2 - g0, 2 * (g1 - g0), atol=1e-12)
+
+
 # ---------------------------------------------------------------- filters
 
 def test_bilateral_properties():

@@ -43,6 +43,11 @@ BATCH_SIZES = ((512, 512), (1280, 720), (1920, 1080), (2560, 1440))
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-sample-px", type=int, default=96 * 96)
     ap.add_argument("--no-denoised", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only launch a 256 MiB device copy 3x and the 1080p gather 3x (for rocprofv3 --pmc)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_1080p_plain_traffic.md"))
     ap.add_argument("--quality-seeds", default=",".join(str(s) for s in QUALITY_SEEDS))
     ap.add_argument("--weights", default=None)
     return ap.parse_args()
@@ -73,6 +81,9 @@ def main():
     scene = lgx.make_scene(SEED, 32, 8)
     view, proj = lgx.scene_matrices(scene, W, H)
     g = dev.GBufferDev.alloc(W, H, "mixed").render(scene)
+    if args.traffic_probe:
+        traffic_probe(g, view, proj, sampler)
+        return
     # The drop-in path: SsvgiRenderer passes declared on the C++ RenderGraph,
     # timed per pass with hipEvents on the frame's stream.
     renderer = dev.Renderer(W, H, DIRS, STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_BILATERAL,
@@ -101,6 +112,8 @@ def main():
     gather_ms = sum(p["ms"] for p in passes if p["name"] == "IndirectLightPass")
     achieved = gather_bytes / (gather_ms * 1e-3) / 1e9
 
+    log(f"plain path: {ms_per_step:.3f} ms/frame, gather {gather_ms:.3f} ms")
+    traffic = read_traffic(args.traffic_file)
     result = {
         "metric": "GI ms/frame @1080p (plain 32-direction HIP path)",
         "value": round(ms_per_step, 4),
@@ -118,7 +131,7 @@ def main():
                                "2 bilateral passes r=4; depth FP32, normal/radiance FP16, albedo RGBA8",
                    "width": W, "height": H, "dirs": DIRS, "max_steps": STEPS, "levels": LEVELS, "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "gather_kernel (IndirectLightPass)", "kernel_ms": round(gather_ms, 4),
                      "algorithmic_bytes": gather_bytes, "steps_executed": total_steps},
         "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(passes)},
@@ -127,14 +140,55 @@ def main():
     if not args.no_denoised:
         seeds = [int(s) for s in args.quality_seeds.split(",") if s]
         result["denoised"] = denoised_leg(args, g, view, proj, sampler, ms_per_step, seeds)
+        log(f"denoised path: {result['denoised']['ms_per_frame']:.3f} ms/frame, "
+            f"rel-L2 {result['denoised']['rel_l2_vs_fp64_256']:.4f}")
 
     if not args.no_batch:
         result["uneven_batch"] = batch_leg(args, sampler)
+        log(f"uneven batch: {result['uneven_batch']['mpixel_per_s']} Mpixel/s")
 
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(g, view, proj, sampler, renderer.intermediate("indirect"),
                                               args.cpu_sample_px)
     print(json.dumps(result), flush=True)
+
+
+COPY_BYTES = 256 << 20
+
+
+def traffic_probe(g, view, proj, sampler):
+    """Kernels for the HBM-traffic counters: a plain device copy of known size
+    (the calibration) and the bench's gather launch, three times each."""
+    import torch
+    from legitengine_amd import device as dev
+    from legitengine_amd import lgx
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    pyr = dev.PyramidDev(g, LEVELS, sampler).build()
+    ga = dev.Gather(W, H, DIRS, STEPS, sampler)
+    out = dev.DImage(W, H, lgx.RGBA32F)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ga(view, proj, g, pyr, out=out)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "gather_launches": 3}),
+          flush=True)
+
+
+def read_traffic(path):
+    """HBM bytes per gather launch from the committed counter summary (the line
+    'traffic_bytes_per_launch: N'), or None."""
+    try:
+        for line in open(path):
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1].strip()))
+    except OSError:
+        return None
+    return None
 
 
 def time_renderer(r, g, view, proj, steps, warmup):
@@ -183,19 +237,22 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         raise FileNotFoundError(f"trained U-Net weights not found at {path}")
     net = dev.UNet(1, H, W)
     net.load(path)
+    unet_tune = net.autotune(1, H, W)
     rden = dev.Renderer(W, H, DEN_DIRS, STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_UNET, kind="mixed",
-                        profile=True)
+                        profile=True, footprint_dirs=dn.NOISY_FOOTPRINT)
     rden.attach_unet(net)
     den_ms, den_passes = time_renderer(rden, g, view, proj, args.steps, args.warmup)
+    log(f"denoised renderer timed: {den_ms:.3f} ms/frame")
     unet_ms = sum(p["ms"] for p in den_passes if p["name"] == "DenoiserPass")
     flops = dn.unet_flops_per_pixel() * W * H
     plain = {}
     for d in PLAIN_SWEEP:
         for kind, den in (("plain", lgx.DENOISER_BILATERAL), ("raw", lgx.DENOISER_NONE)):
-            r = dev.Renderer(W, H, d, STEPS, LEVELS, sampler, GRID, denoiser=den, denoise_passes=BILAT_PASSES,
-                             denoise_radius=BILAT_RADIUS, kind="mixed", profile=True)
+            r = dev.Renderer(W, H, d, STEPS if d <= DIRS else 0, LEVELS, sampler, GRID, denoiser=den,
+                             denoise_passes=BILAT_PASSES, denoise_radius=BILAT_RADIUS, kind="mixed", profile=True)
             ms, _ = time_renderer(r, g, view, proj, max(3, args.steps // 4), 1)
             plain[f"{kind}{d}"] = {"renderer": r, "ms": ms, "err": []}
+            log(f"{kind}{d}: {ms:.3f} ms/frame")
     den_err, noisy_err = [], []
     gs = dev.GBufferDev.alloc(W, H, "mixed")
     target = dev.Gather(W, H, TARGET_DIRS, 0, sampler, precision=lgx.PREC_F64)
@@ -211,6 +268,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         for e in plain.values():
             e["renderer"].render(gs, v, p)
             e["err"].append(rel_l2(e["renderer"].output(), tgt))
+        log(f"seed {s}: denoised rel-L2 {den_err[-1]:.4f}")
     err = float(np.mean(den_err))
     sweep = {k: {"ms_per_frame": round(e["ms"], 4), "rel_l2": round(float(np.mean(e["err"])), 5)}
              for k, e in plain.items()}
@@ -221,8 +279,9 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     for e in plain.values():
         e["renderer"].close()
     return {
-        "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather, {STEPS} steps, {LEVELS}-level Hi-Z, 4x4 "
-                    "interleave, BF16 U-Net denoiser pass (3 levels, 32/64/128 channels, FP32 accumulation)",
+        "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather (step growth and lod of {dn.NOISY_FOOTPRINT} "
+                    f"directions), {STEPS} steps, {LEVELS}-level Hi-Z, 4x4 interleave, BF16 U-Net denoiser pass "
+                    "(3 levels, 32/64/128 channels, FP32 accumulation)",
         "ms_per_frame": round(den_ms, 4),
         "unet_ms": round(unet_ms, 4),
         "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(den_passes)},
@@ -232,6 +291,8 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         "target": f"{TARGET_DIRS}-direction gather, reference step rule (no cap), FP64 on the GPU",
         "seeds": seeds,
         "plain_sweep": sweep,
+        "plain_sweep_note": "plain path = 2 bilateral passes r=4, raw = no denoiser; 64 steps at 32 directions, "
+                            "the reference step rule without cap above",
         "plain_ms_at_equal_error": round(eq_ms, 4) if eq_ms else None,
         "speedup_at_equal_error": round(eq_ms / den_ms, 3) if eq_ms else None,
         "speedup_vs_plain32": round(plain32_ms / den_ms, 3),
@@ -240,6 +301,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
                           "frac": round(flops / (unet_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
                           "flops_per_frame": flops},
         "weights": os.path.relpath(path, ROOT),
+        "unet_tiles": [r["best"] for r in unet_tune["forward"]],
     }
 
 
@@ -269,9 +331,12 @@ def batch_leg(args, sampler):
     net = dev.UNet(1, maxh, maxw)
     path = args.weights or dn.WEIGHTS
     net.load(path)
+    net.autotune(1, 1080, 1920)  # register tiles at the most common extent
     b = dev.Batch(views, dirs=DEN_DIRS, max_steps=STEPS, levels=LEVELS, sampler=sampler, grid=GRID,
                   footprint_dirs=dn.NOISY_FOOTPRINT, net=net)
+    log(f"batch of {len(views)} views created")
     tune = b.autotune()
+    log(f"autotune best {tune['best']}")
     stream = torch.cuda.Stream()
     b.capture(stream)
     for _ in range(max(1, args.warmup)):

@@ -280,6 +280,18 @@ int lgx_unet_get_state(lgx_unet *u, int32_t which, float *host, int64_t n, int32
 int lgx_unet_set_state(lgx_unet *u, int32_t which, const float *host, int64_t n, int32_t adam_t);
 /* Inference from a G-buffer and the noisy irradiance (RGBA32F); out RGBA32F (any pitch). */
 int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_image *noisy, lgx_image *out, lgx_stream_t stream);
+/* Times the register tiles of every convolution at n x h x w (forward, and
+ * the input gradients when backward != 0) on the network's own buffers,
+ * keeps the fastest per layer and writes the table as JSON to buf.  The
+ * network's activations are overwritten. */
+int lgx_unet_autotune(lgx_unet *u, int32_t n, int32_t h, int32_t w, int32_t backward, lgx_stream_t stream,
+                      char *buf, size_t cap);
+/* Register tiles offered for a convolution with cout output channels, and
+ * the tile index of each of the 10 convolutions (forward fwd[l], input
+ * gradient dx[l]; dx[0] is unused); every tile gives bit-identical results. */
+int lgx_unet_conv_tiles(int32_t cout);
+int lgx_unet_set_tiles(lgx_unet *u, const int32_t *fwd, const int32_t *dx);
+int lgx_unet_get_tiles(const lgx_unet *u, int32_t *fwd, int32_t *dx);
 /* One wave computes D(16x16) = A(16x32, row-major) * B(32x16, row-major) on the
  * matrix core through the operand layout the convolutions use (device FP32
  * pointers; operands rounded to BF16). */

@@ -24,6 +24,7 @@ struct ConvArgs {
   int split;
   bf16 *out1;     // channels [split, cout)
   const bf16 *mask0, *mask1;
+  int tile;       // register tile: an index below conv3x3_tiles(cout), 0 = default
 };
 
 struct WgradArgs {
@@ -39,6 +40,7 @@ struct WgradArgs {
 };
 
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
+int conv3x3_tiles(int cout);  // register tiles offered for this output channel count
 hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s);
 hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s);
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);

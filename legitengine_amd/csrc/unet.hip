@@ -14,7 +14,10 @@
 // depth, octahedral normal xy, albedo rgb.
 #include <math.h>
 
+#include <cstdio>
+#include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "lgx_internal.h"
@@ -66,6 +69,9 @@ class Net {
   bf16 *g_d1, *g_d1a, *g_upd2, *g_e1c, *g_d2, *g_d2a, *g_upb2, *g_e2c, *g_b2, *g_b1, *g_p2, *g_e2, *g_e2a, *g_p1, *g_e1,
       *g_e1a;
   int adam_t = 0;
+  // register tile of each layer's forward and input-gradient convolution
+  // (an index below conv3x3_tiles(cout)); set by autotune()
+  int fwd_tile[10] = {}, dx_tile[10] = {};
 
   void *alloc(size_t bytes) {
     std::unique_ptr<DevBuf> b(new DevBuf);
@@ -172,6 +178,7 @@ class Net {
     c.relu = 1;
     c.out0 = out_;
     c.split = c.cout;
+    c.tile = fwd_tile[l];
     check(conv3x3(c, s), "conv3x3");
   }
 
@@ -191,6 +198,7 @@ class Net {
     c.out1 = o1;
     c.mask0 = m0;
     c.mask1 = m1;
+    c.tile = dx_tile[l];
     check(conv3x3(c, s), "conv3x3 input gradient");
   }
 
@@ -216,25 +224,106 @@ class Net {
     check(channel_sum(dy, kLayers[l].cout, (size_t)n * h * w, grads + b_off[l], s), "channel_sum");
   }
 
+  // forward convolution l on the activations of an n x h x w input
+  void fwd_layer(int l, int n, int h, int w, hipStream_t s) {
+    const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
+    switch (l) {
+      case 0: conv(0, x32, 32, 0, nullptr, 0, n, h, w, e1a, s); break;
+      case 1: conv(1, e1a, 32, 0, nullptr, 0, n, h, w, e1, s); break;
+      case 2: conv(2, p1, 32, 0, nullptr, 0, n, h2, w2, e2a, s); break;
+      case 3: conv(3, e2a, 64, 0, nullptr, 0, n, h2, w2, e2, s); break;
+      case 4: conv(4, p2, 64, 0, nullptr, 0, n, h4, w4, b1, s); break;
+      case 5: conv(5, b1, 128, 0, nullptr, 0, n, h4, w4, b2, s); break;
+      case 6: conv(6, b2, 128, 1, e2, 64, n, h2, w2, d2a, s); break;  // [up(b2), e2]
+      case 7: conv(7, d2a, 64, 0, nullptr, 0, n, h2, w2, d2, s); break;
+      case 8: conv(8, d2, 64, 1, e1, 32, n, h, w, d1a, s); break;  // [up(d2), e1]
+      default: conv(9, d1a, 32, 0, nullptr, 0, n, h, w, d1, s); break;
+    }
+  }
+
+  // input gradient of convolution l (1..9) in the backward pass
+  void dx_layer(int l, int n, int h, int w, hipStream_t s) {
+    const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
+    switch (l) {
+      case 9: conv_dx(9, g_d1, n, h, w, g_d1a, 32, nullptr, d1a, nullptr, s); break;
+      case 8: conv_dx(8, g_d1a, n, h, w, g_upd2, 64, g_e1c, nullptr, nullptr, s); break;
+      case 7: conv_dx(7, g_d2, n, h2, w2, g_d2a, 64, nullptr, d2a, nullptr, s); break;
+      case 6: conv_dx(6, g_d2a, n, h2, w2, g_upb2, 128, g_e2c, nullptr, nullptr, s); break;
+      case 5: conv_dx(5, g_b2, n, h4, w4, g_b1, 128, nullptr, b1, nullptr, s); break;
+      case 4: conv_dx(4, g_b1, n, h4, w4, g_p2, 64, nullptr, nullptr, nullptr, s); break;
+      case 3: conv_dx(3, g_e2, n, h2, w2, g_e2a, 64, nullptr, e2a, nullptr, s); break;
+      case 2: conv_dx(2, g_e2a, n, h2, w2, g_p1, 32, nullptr, nullptr, nullptr, s); break;
+      default: conv_dx(1, g_e1, n, h, w, g_e1a, 32, nullptr, e1a, nullptr, s); break;
+    }
+  }
+
   void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
     const size_t P = (size_t)n * h * w;
-    const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
+    const int h2 = h / 2, w2 = w / 2;
     check(pack_input(x12, P, x32, s), "pack_input");
-    conv(0, x32, 32, 0, nullptr, 0, n, h, w, e1a, s);
-    conv(1, e1a, 32, 0, nullptr, 0, n, h, w, e1, s);
+    fwd_layer(0, n, h, w, s);
+    fwd_layer(1, n, h, w, s);
     check(avgpool(e1, n, h, w, 32, p1, s), "avgpool");
-    conv(2, p1, 32, 0, nullptr, 0, n, h2, w2, e2a, s);
-    conv(3, e2a, 64, 0, nullptr, 0, n, h2, w2, e2, s);
+    fwd_layer(2, n, h, w, s);
+    fwd_layer(3, n, h, w, s);
     check(avgpool(e2, n, h2, w2, 64, p2, s), "avgpool");
-    conv(4, p2, 64, 0, nullptr, 0, n, h4, w4, b1, s);
-    conv(5, b1, 128, 0, nullptr, 0, n, h4, w4, b2, s);
-    conv(6, b2, 128, 1, e2, 64, n, h2, w2, d2a, s);
-    conv(7, d2a, 64, 0, nullptr, 0, n, h2, w2, d2, s);
-    conv(8, d2, 64, 1, e1, 32, n, h, w, d1a, s);
-    conv(9, d1a, 32, 0, nullptr, 0, n, h, w, d1, s);
+    for (int l = 4; l < 10; l++) fwd_layer(l, n, h, w, s);
     check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, w, out_pitch > 0 ? out_pitch : w,
                    (float4 *)out_, s),
           "head_fwd");
+  }
+
+  // Times every register tile of every forward (and, with backward, every
+  // input-gradient) convolution at this shape on the network's own buffers
+  // and keeps the fastest; returns the table as JSON.
+  std::string autotune(int n, int h, int w, bool with_backward, hipStream_t s) {
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    std::string js = "{\"forward\": [";
+    auto run = [&](int *tiles, int l, bool dx) {
+      const int cout = dx ? kLayers[l].cin_pad : kLayers[l].cout;
+      const int nt = conv3x3_tiles(cout);
+      float best = 1e30f;
+      int bt = 0;
+      std::string row = "[";
+      for (int t = 0; t < nt; t++) {
+        tiles[l] = t;
+        auto launch = [&]() {
+          if (dx)
+            dx_layer(l, n, h, w, s);
+          else
+            fwd_layer(l, n, h, w, s);
+        };
+        launch();
+        check(hipEventRecord(e0, s), "hipEventRecord");
+        for (int r = 0; r < 3; r++) launch();
+        check(hipEventRecord(e1, s), "hipEventRecord");
+        check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        ms /= 3;
+        char b[48];
+        snprintf(b, sizeof(b), "%s%.5f", t ? ", " : "", ms);
+        row += b;
+        if (ms < best) {
+          best = ms;
+          bt = t;
+        }
+      }
+      tiles[l] = bt;
+      char b[64];
+      snprintf(b, sizeof(b), "], \"best\": %d}", bt);
+      return "{\"layer\": " + std::to_string(l) + ", \"ms\": " + row + b;
+    };
+    for (int l = 0; l < 10; l++) js += (l ? ", " : "") + run(fwd_tile, l, false);
+    js += "], \"input_gradient\": [";
+    if (with_backward)
+      for (int l = 1; l < 10; l++) js += (l > 1 ? ", " : "") + run(dx_tile, l, true);
+    js += "]}";
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return js;
   }
 
   void backward(const float *x12, const float *target, int n, int h, int w, hipStream_t s) {
@@ -249,32 +338,32 @@ class Net {
           "head_bwd");
     // d1 = relu(conv9(d1a))
     wgrad(9, d1a, 32, 0, nullptr, 0, g_d1, n, h, w, s);
-    conv_dx(9, g_d1, n, h, w, g_d1a, 32, nullptr, d1a, nullptr, s);
+    dx_layer(9, n, h, w, s);
     // d1a = relu(conv8([up(d2), e1]))
     wgrad(8, d2, 64, 1, e1, 32, g_d1a, n, h, w, s);
-    conv_dx(8, g_d1a, n, h, w, g_upd2, 64, g_e1c, nullptr, nullptr, s);
+    dx_layer(8, n, h, w, s);
     check(upsample_bwd(g_upd2, d2, n, h2, w2, 64, g_d2, s), "upsample_bwd");
     // d2 = relu(conv7(d2a))
     wgrad(7, d2a, 64, 0, nullptr, 0, g_d2, n, h2, w2, s);
-    conv_dx(7, g_d2, n, h2, w2, g_d2a, 64, nullptr, d2a, nullptr, s);
+    dx_layer(7, n, h, w, s);
     // d2a = relu(conv6([up(b2), e2]))
     wgrad(6, b2, 128, 1, e2, 64, g_d2a, n, h2, w2, s);
-    conv_dx(6, g_d2a, n, h2, w2, g_upb2, 128, g_e2c, nullptr, nullptr, s);
+    dx_layer(6, n, h, w, s);
     check(upsample_bwd(g_upb2, b2, n, h4, w4, 128, g_b2, s), "upsample_bwd");
     // b2 = relu(conv5(b1)), b1 = relu(conv4(p2))
     wgrad(5, b1, 128, 0, nullptr, 0, g_b2, n, h4, w4, s);
-    conv_dx(5, g_b2, n, h4, w4, g_b1, 128, nullptr, b1, nullptr, s);
+    dx_layer(5, n, h, w, s);
     wgrad(4, p2, 64, 0, nullptr, 0, g_b1, n, h4, w4, s);
-    conv_dx(4, g_b1, n, h4, w4, g_p2, 64, nullptr, nullptr, nullptr, s);
+    dx_layer(4, n, h, w, s);
     // e2 feeds the pool and the skip
     check(pool_bwd_add(g_e2c, g_p2, e2, n, h2, w2, 64, g_e2, s), "pool_bwd_add");
     wgrad(3, e2a, 64, 0, nullptr, 0, g_e2, n, h2, w2, s);
-    conv_dx(3, g_e2, n, h2, w2, g_e2a, 64, nullptr, e2a, nullptr, s);
+    dx_layer(3, n, h, w, s);
     wgrad(2, p1, 32, 0, nullptr, 0, g_e2a, n, h2, w2, s);
-    conv_dx(2, g_e2a, n, h2, w2, g_p1, 32, nullptr, nullptr, nullptr, s);
+    dx_layer(2, n, h, w, s);
     check(pool_bwd_add(g_e1c, g_p1, e1, n, h, w, 32, g_e1, s), "pool_bwd_add");
     wgrad(1, e1a, 32, 0, nullptr, 0, g_e1, n, h, w, s);
-    conv_dx(1, g_e1, n, h, w, g_e1a, 32, nullptr, e1a, nullptr, s);
+    dx_layer(1, n, h, w, s);
     wgrad(0, x32, 32, 0, nullptr, 0, g_e1a, n, h, w, s);
     for (int l = 0; l < 10; l++)
       check(unpack_wgrad(dw[l], kLayers[l].cout, kLayers[l].cin, kLayers[l].cin_pad, grads + w_off[l], s),
@@ -436,6 +525,42 @@ extern "C" int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_ima
     hipStream_t s = (hipStream_t)stream;
     N.check(lgx::unet::pack_gbuffer(*noisy, *g, N.x12_scratch, s), "pack_gbuffer");
     N.forward(N.x12_scratch, 1, noisy->h, noisy->w, (float *)out->ptr, s, out->pitch);
+  });
+}
+
+extern "C" int lgx_unet_conv_tiles(int32_t cout) { return lgx::unet::conv3x3_tiles(cout); }
+
+extern "C" int lgx_unet_set_tiles(lgx_unet *u, const int32_t *fwd, const int32_t *dx) {
+  if (!u || !u->net || !fwd || !dx) return LGX_EINVAL;
+  for (int l = 0; l < 10; l++) {
+    if (fwd[l] < 0 || fwd[l] >= lgx::unet::conv3x3_tiles(lgx::unet::kLayers[l].cout)) return LGX_EINVAL;
+    if (dx[l] < 0 || dx[l] >= lgx::unet::conv3x3_tiles(lgx::unet::kLayers[l].cin_pad)) return LGX_EINVAL;
+  }
+  for (int l = 0; l < 10; l++) {
+    u->net->fwd_tile[l] = fwd[l];
+    u->net->dx_tile[l] = dx[l];
+  }
+  return LGX_OK;
+}
+
+extern "C" int lgx_unet_get_tiles(const lgx_unet *u, int32_t *fwd, int32_t *dx) {
+  if (!u || !u->net || !fwd || !dx) return LGX_EINVAL;
+  for (int l = 0; l < 10; l++) {
+    fwd[l] = u->net->fwd_tile[l];
+    dx[l] = u->net->dx_tile[l];
+  }
+  return LGX_OK;
+}
+
+extern "C" int lgx_unet_autotune(lgx_unet *u, int32_t n, int32_t h, int32_t w, int32_t backward, lgx_stream_t stream,
+                                 char *buf, size_t cap) {
+  if (!shape_ok(u, n, h, w)) return LGX_EINVAL;
+  return unet_guard([&]() {
+    std::string js = u->net->autotune(n, h, w, backward != 0, (hipStream_t)stream);
+    if (buf && cap) {
+      if (js.size() + 1 > cap) throw Net::Error(LGX_EINVAL, "lgx_unet_autotune: buffer too small");
+      memcpy(buf, js.c_str(), js.size() + 1);
+    }
   });
 }
 

@@ -60,77 +60,120 @@ LGX_D bf16x8 load8(const ConvInput &in, int n, int y, int x, int h, int w, int c
 }
 
 // Forward 3x3 convolution (stride 1, zero padding 1) + bias + ReLU, and the
-// same kernel with flipped/transposed weights for the input gradient.  A
-// workgroup is 4 waves on 4 consecutive rows; a wave computes 64 pixels of
-// its row (4 blocks of 16) for all output channels (MB blocks of 16).
+// same kernel with flipped/transposed weights for the input gradient.
+// A wave computes R output rows x PB blocks of 16 pixels x all output
+// channels (MB blocks of 16); the 4 waves of a workgroup sit side by side
+// along x and share the weight fragments through L1.  Per 32-channel input
+// chunk and horizontal tap offset dx, the wave loads the R + 2 input rows it
+// needs once and feeds each of them to the three vertical taps, so an
+// activation fragment serves up to 3 * MB MFMAs and a weight fragment R * PB.
 // Output channels [0, split) go to out0, [split, cout) to out1 (the two
 // halves of a concatenated input when computing its gradient); an optional
 // mask zeroes outputs whose mask activation is <= 0 (the ReLU derivative of
 // the tensor the gradient flows into).
-template <int MB, int PB> __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a) {
+template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int n = blockIdx.z, y = blockIdx.y * 4 + wave, x0 = blockIdx.x * (PB * 16);
-  if (y >= a.h) return;
+  const int n = blockIdx.z, y0 = blockIdx.y * R, x0 = (blockIdx.x * 4 + wave) * (PB * 16);
+  if (x0 >= a.w) return;
   const int col = lane & 15, kg = lane >> 4;
-  const ConvInput in{a.a, a.ca, a.a_up, a.b, a.cb};
   const int cin = a.ca + a.cb, chunks = cin >> 5;
-  f32x4 acc[MB][PB];
+  f32x4 acc[R][PB][MB];
 #pragma unroll
-  for (int mb = 0; mb < MB; mb++)
+  for (int r = 0; r < R; r++)
 #pragma unroll
-    for (int pb = 0; pb < PB; pb++) acc[mb][pb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int tap = 0; tap < 9; tap++) {
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    for (int cc = 0; cc < chunks; cc++) {
-      bf16x8 bx[PB];
+    for (int pb = 0; pb < PB; pb++)
 #pragma unroll
-      for (int pb = 0; pb < PB; pb++) bx[pb] = load8(in, n, y + dy, x0 + pb * 16 + col + dx, a.h, a.w, cc * 32 + kg * 8);
-      const bf16 *wrow = a.wt + ((size_t)(tap * chunks + cc) * a.cout + col) * 32 + kg * 8;
+      for (int mb = 0; mb < MB; mb++) acc[r][pb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int cc = 0; cc < chunks; cc++) {
+    // the chunk's source is uniform: A (optionally at half resolution) or B
+    const bool from_a = cc * 32 < a.ca;
+    const bool up = from_a && a.a_up;
+    const int cs = from_a ? a.ca : a.cb;
+    const int sw = up ? (a.w >> 1) : a.w, sh = up ? (a.h >> 1) : a.h;
+    const bf16 *img = (from_a ? a.a : a.b) + (size_t)n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca) + kg * 8;
+    const bf16 *wchunk = a.wt + ((size_t)cc * a.cout + col) * 32 + kg * 8;
+    const size_t wtap = (size_t)chunks * a.cout * 32;
+#pragma unroll 1
+    for (int dx = -1; dx <= 1; dx++) {
+      uint32_t xoff[PB];
+      bool xok[PB];
 #pragma unroll
-      for (int mb = 0; mb < MB; mb++) {
-        const bf16x8 aw = *(const bf16x8 *)(wrow + (size_t)mb * 16 * 32);
+      for (int pb = 0; pb < PB; pb++) {
+        const int x = x0 + pb * 16 + col + dx;
+        xok[pb] = x >= 0 && x < a.w;
+        xoff[pb] = (uint32_t)(up ? (x >> 1) : x) * (uint32_t)cs;
+      }
+      bf16x8 bx[R + 2][PB];
 #pragma unroll
-        for (int pb = 0; pb < PB; pb++) acc[mb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw, bx[pb], acc[mb][pb], 0, 0, 0);
+      for (int rr = 0; rr < R + 2; rr++) {
+        const int yy = y0 - 1 + rr;
+        const bool yok = yy >= 0 && yy < a.h;
+        const uint32_t roff = (uint32_t)((up ? (yy >> 1) : yy) * sw) * (uint32_t)cs;
+#pragma unroll
+        for (int pb = 0; pb < PB; pb++) {
+          bf16x8 z = {};
+          bx[rr][pb] = (yok && xok[pb]) ? *(const bf16x8 *)(img + roff + xoff[pb]) : z;
+        }
+      }
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        const bf16 *wrow = wchunk + (size_t)((dy + 1) * 3 + (dx + 1)) * wtap;
+        bf16x8 aw[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) aw[mb] = *(const bf16x8 *)(wrow + (size_t)mb * 16 * 32);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int pb = 0; pb < PB; pb++)
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+              acc[r][pb][mb] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[mb], bx[r + 1 + dy][pb], acc[r][pb][mb], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int pb = 0; pb < PB; pb++) {
-    const int x = x0 + pb * 16 + col;
-    if (x >= a.w) continue;
-    const size_t pix = ((size_t)n * a.h + y) * a.w + x;
+  for (int r = 0; r < R; r++) {
+    const int y = y0 + r;
+    if (y >= a.h) break;
 #pragma unroll
-    for (int mb = 0; mb < MB; mb++) {
-      const int co = mb * 16 + kg * 4;
-      float v[4];
+    for (int pb = 0; pb < PB; pb++) {
+      const int x = x0 + pb * 16 + col;
+      if (x >= a.w) continue;
+      const size_t pix = ((size_t)n * a.h + y) * a.w + x;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        v[i] = acc[mb][pb][i];
-        if (a.bias) v[i] += a.bias[co + i];
-        if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
-      }
-      bf16 *dst;
-      const bf16 *mask;
-      int stride, c;
-      if (co < a.split) {
-        dst = a.out0;
-        mask = a.mask0;
-        stride = a.split;
-        c = co;
-      } else {
-        dst = a.out1;
-        mask = a.mask1;
-        stride = a.cout - a.split;
-        c = co - a.split;
-      }
-      if (mask) {
+      for (int mb = 0; mb < MB; mb++) {
+        const int co = mb * 16 + kg * 4;
+        float v[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-          if (!(bf2f(mask[pix * stride + c + i]) > 0.f)) v[i] = 0.f;
+        for (int i = 0; i < 4; i++) {
+          v[i] = acc[r][pb][mb][i];
+          if (a.bias) v[i] += a.bias[co + i];
+          if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
+        }
+        bf16 *dst;
+        const bf16 *mask;
+        int stride, cch;
+        if (co < a.split) {
+          dst = a.out0;
+          mask = a.mask0;
+          stride = a.split;
+          cch = co;
+        } else {
+          dst = a.out1;
+          mask = a.mask1;
+          stride = a.cout - a.split;
+          cch = co - a.split;
+        }
+        if (mask) {
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            if (!(bf2f(mask[pix * stride + cch + i]) > 0.f)) v[i] = 0.f;
+        }
+        typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
+        *(bf16x4 *)(dst + pix * stride + cch) = o;
       }
-      typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
-      bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
-      *(bf16x4 *)(dst + pix * stride + c) = o;
     }
   }
 }
@@ -468,21 +511,82 @@ static unsigned grid_for(size_t work, size_t per_block) {
   return (unsigned)(g ? g : 1);
 }
 
-// Pixel blocks per wave: 4 while the accumulators fit comfortably, 2 for
-// the wide layers so two waves per SIMD stay resident.
-hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
-  const int mb = a.cout / 16, pb = mb >= 6 ? 2 : 4;
-  dim3 grid((a.w + pb * 16 - 1) / (pb * 16), (a.h + 3) / 4, a.n), block(256);
+// Register tiles (R rows, PB pixel blocks) offered per output-channel block
+// count MB, accumulators R * PB * MB * 4 VGPRs <= 128; entry 0 is the
+// default, the others are the autotuner's candidates.
+struct ConvTile {
+  int r, pb;
+};
+static const ConvTile kTiles1[] = {{4, 2}, {2, 4}, {4, 4}, {1, 4}};
+static const ConvTile kTiles2[] = {{4, 2}, {2, 4}, {4, 4}, {1, 4}};
+static const ConvTile kTiles4[] = {{2, 2}, {4, 2}, {2, 4}, {1, 4}};
+static const ConvTile kTiles6[] = {{2, 2}, {1, 4}, {1, 2}};
+static const ConvTile kTiles8[] = {{1, 2}, {2, 2}, {1, 4}};
+static const ConvTile kTiles12[] = {{1, 2}, {2, 1}};
+
+static const ConvTile *conv_tiles(int mb, int *count) {
+#define LGX_TILES(T) \
+  *count = (int)(sizeof(T) / sizeof(T[0])); \
+  return T
   switch (mb) {
-    case 1: hipLaunchKernelGGL((conv3x3_kernel<1, 4>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((conv3x3_kernel<2, 4>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((conv3x3_kernel<4, 4>), grid, block, 0, s, a); break;
-    case 6: hipLaunchKernelGGL((conv3x3_kernel<6, 2>), grid, block, 0, s, a); break;
-    case 8: hipLaunchKernelGGL((conv3x3_kernel<8, 2>), grid, block, 0, s, a); break;
-    case 12: hipLaunchKernelGGL((conv3x3_kernel<12, 2>), grid, block, 0, s, a); break;
+    case 1: LGX_TILES(kTiles1);
+    case 2: LGX_TILES(kTiles2);
+    case 4: LGX_TILES(kTiles4);
+    case 6: LGX_TILES(kTiles6);
+    case 8: LGX_TILES(kTiles8);
+    case 12: LGX_TILES(kTiles12);
+    default: *count = 0; return nullptr;
+  }
+#undef LGX_TILES
+}
+
+int conv3x3_tiles(int cout) {
+  int n = 0;
+  if (cout % 16) return 0;
+  (void)conv_tiles(cout / 16, &n);
+  return n;
+}
+
+template <int MB, int PB, int R> static void launch_conv(const ConvArgs &a, hipStream_t s) {
+  dim3 grid((a.w + 4 * PB * 16 - 1) / (4 * PB * 16), (a.h + R - 1) / R, a.n), block(256);
+  hipLaunchKernelGGL((conv3x3_kernel<MB, PB, R>), grid, block, 0, s, a);
+}
+
+template <int MB> static hipError_t conv_mb(const ConvArgs &a, const ConvTile &t, hipStream_t s) {
+#define LGX_CASE(RR, PP) \
+  if (t.r == RR && t.pb == PP) { \
+    launch_conv<MB, PP, RR>(a, s); \
+    return hipGetLastError(); \
+  }
+  if constexpr (MB <= 2) {
+    LGX_CASE(4, 2) LGX_CASE(2, 4) LGX_CASE(4, 4) LGX_CASE(1, 4)
+  } else if constexpr (MB <= 4) {
+    LGX_CASE(2, 2) LGX_CASE(4, 2) LGX_CASE(2, 4) LGX_CASE(1, 4)
+  } else if constexpr (MB <= 8) {
+    LGX_CASE(2, 2) LGX_CASE(1, 4) LGX_CASE(1, 2)  // covers the MB = 6 and MB = 8 tables
+  } else {
+    LGX_CASE(1, 2) LGX_CASE(2, 1)
+  }
+#undef LGX_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
+  if (a.cout % 16) return hipErrorInvalidValue;
+  const int mb = a.cout / 16;
+  int n = 0;
+  const ConvTile *tiles = conv_tiles(mb, &n);
+  if (!tiles || a.tile < 0 || a.tile >= n) return hipErrorInvalidValue;
+  const ConvTile &t = tiles[a.tile];
+  switch (mb) {
+    case 1: return conv_mb<1>(a, t, s);
+    case 2: return conv_mb<2>(a, t, s);
+    case 4: return conv_mb<4>(a, t, s);
+    case 6: return conv_mb<6>(a, t, s);
+    case 8: return conv_mb<8>(a, t, s);
+    case 12: return conv_mb<12>(a, t, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s) {

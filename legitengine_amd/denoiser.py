@@ -27,7 +27,7 @@ W, H = 1920, 1080
 LEVELS = 6
 CROP = 256
 NOISY_DIRS, TARGET_DIRS, PLAIN_DIRS, STEPS = 4, 256, 32, 64
-NOISY_FOOTPRINT = 0
+NOISY_FOOTPRINT = 64  # the input marches with the step growth and lod of 64 directions
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_1080p.safetensors")
 # (logical cin, cout, resolution divisor) of the ten 3x3 convolutions; the
 # 1x1 head maps 32 channels to 3
@@ -203,6 +203,7 @@ def main(argv=None):
         net = dev.UNet(args.batch, CROP, CROP, seed=0)
         if args.resume:
             net.load(args.resume)
+        net.autotune(args.batch, CROP, CROP, backward=True)
         hist, secs = train(net, X, T, args.steps, args.batch, args.lr, log=log, schedule=args.schedule)
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
         net.save(args.out)
@@ -210,6 +211,7 @@ def main(argv=None):
         del X, T
         infer = dev.UNet(1, H, W)
         infer.load(args.out)
+        infer.autotune(1, H, W)
         rows = evaluate(infer, [int(s) for s in args.eval_seeds.split(",") if s], footprint=args.footprint)
         if args.report:
             with open(args.report, "w") as f:

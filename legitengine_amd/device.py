@@ -284,6 +284,23 @@ class UNet:
         lgx.check(lgx.lib().lgx_unet_train_step(self.h, C.c_void_p(x12.data_ptr()), C.c_void_p(target.data_ptr()), n,
                                                 h, w, C.c_float(lr), lp, stream_handle(stream)), "lgx_unet_train_step")
 
+    def tiles(self):
+        f, d = (C.c_int32 * 10)(), (C.c_int32 * 10)()
+        lgx.check(lgx.lib().lgx_unet_get_tiles(self.h, f, d), "lgx_unet_get_tiles")
+        return list(f), list(d)
+
+    def set_tiles(self, fwd, dx):
+        lgx.check(lgx.lib().lgx_unet_set_tiles(self.h, (C.c_int32 * 10)(*fwd), (C.c_int32 * 10)(*dx)),
+                  "lgx_unet_set_tiles")
+
+    def autotune(self, n, h, w, backward=False, stream=None):
+        """Picks each convolution's register tile at this shape (overwrites activations)."""
+        import json
+        buf = C.create_string_buffer(1 << 14)
+        lgx.check(lgx.lib().lgx_unet_autotune(self.h, n, h, w, int(backward), stream_handle(stream), buf, len(buf)),
+                  "lgx_unet_autotune")
+        return json.loads(buf.value.decode())
+
     def get(self, which=0):
         a = np.zeros(self.n_params, np.float32)
         t = C.c_int32()

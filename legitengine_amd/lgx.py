@@ -124,6 +124,11 @@ SIGNATURES = {
     "lgx_unet_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]),
     "lgx_unet_set_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32]),
     "lgx_unet_denoise": (C.c_int, [C.c_void_p, C.POINTER(GBuffer), C.POINTER(Image), C.POINTER(Image), C.c_void_p]),
+    "lgx_unet_conv_tiles": (C.c_int, [C.c_int32]),
+    "lgx_unet_set_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lgx_unet_get_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lgx_unet_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_char_p,
+                                    C.c_size_t]),
     "lgx_unet_mfma_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lgx_unet_destroy": (None, [C.c_void_p]),
     "lgx_gather_batch_create": (C.c_int, [C.POINTER(GatherParams), C.POINTER(C.c_void_p), C.POINTER(GBuffer),

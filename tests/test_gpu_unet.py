@@ -147,3 +147,33 @@ def test_denoise_from_gbuffer_matches_records(dev):
     ref = net.forward(torch.from_numpy(rec).cuda())
     torch.cuda.synchronize()
     assert np.allclose(out.numpy(), ref.cpu().numpy()[0], atol=1e-6)
+
+
+def test_register_tiles_bit_identical_and_autotune(dev):
+    """Every convolution register tile accumulates each output in the same
+    order, so forward outputs are bit-identical whatever the tiles, and so
+    are the activation gradients; the weight gradients differ only by the
+    order of their FP32 atomic sums.  The autotuner only picks among them."""
+    from legitengine_amd import lgx
+    flat, x12, target = case(5, n=2, h=48, w=80)
+    net = dev.UNet(2, 48, 80)
+    net.set(0, flat)
+    x, t = torch.from_numpy(x12).cuda(), torch.from_numpy(target).cuda()
+    couts = [co for _, co in U.LAYERS]
+    cins = [32, 32, 32, 64, 64, 128, 192, 64, 96, 32]  # padded input channels
+    ref_out = net.forward(x).cpu().numpy()
+    net.backward(x, t)
+    ref_grad = net.get(1)[0]
+    counts_f = [lgx.lib().lgx_unet_conv_tiles(c) for c in couts]
+    counts_d = [lgx.lib().lgx_unet_conv_tiles(c) for c in cins]
+    assert min(counts_f) >= 2 and min(counts_d) >= 2
+    for k in range(1, max(counts_f + counts_d)):
+        net.set_tiles([k % c for c in counts_f], [k % c for c in counts_d])
+        assert net.forward(x).cpu().numpy().tobytes() == ref_out.tobytes(), k
+        net.backward(x, t)
+        assert rel(net.get(1)[0], ref_grad) < 1e-5, k
+    rep = net.autotune(2, 48, 80, backward=True)
+    f, d = net.tiles()
+    assert [r["best"] for r in rep["forward"]] == f
+    assert [r["best"] for r in rep["input_gradient"]] == d[1:]
+    assert net.forward(x).cpu().numpy().tobytes() == ref_out.tobytes()

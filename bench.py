@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--cpu-sample-px", type=int, default=96 * 96)
     ap.add_argument("--no-denoised", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-quality", action="store_true",
+                    help="time the denoised path only (no error sweep against the FP64 target)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch a 256 MiB device copy 3x and the 1080p gather 3x (for rocprofv3 --pmc)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_1080p_plain_traffic.md"))
@@ -141,7 +143,7 @@ def main():
         seeds = [int(s) for s in args.quality_seeds.split(",") if s]
         result["denoised"] = denoised_leg(args, g, view, proj, sampler, ms_per_step, seeds)
         log(f"denoised path: {result['denoised']['ms_per_frame']:.3f} ms/frame, "
-            f"rel-L2 {result['denoised']['rel_l2_vs_fp64_256']:.4f}")
+            f"rel-L2 {result['denoised'].get('rel_l2_vs_fp64_256')}")
 
     if not args.no_batch:
         result["uneven_batch"] = batch_leg(args, sampler)
@@ -245,6 +247,18 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     log(f"denoised renderer timed: {den_ms:.3f} ms/frame")
     unet_ms = sum(p["ms"] for p in den_passes if p["name"] == "DenoiserPass")
     flops = dn.unet_flops_per_pixel() * W * H
+    timing = {
+        "ms_per_frame": round(den_ms, 4),
+        "unet_ms": round(unet_ms, 4),
+        "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(den_passes)},
+        "unet_roofline": {"bound": "mfma", "achieved": round(flops / (unet_ms * 1e-3) / 1e12, 2),
+                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(flops / (unet_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
+                          "flops_per_frame": flops},
+        "unet_tiles": [r["best"] for r in unet_tune["forward"]],
+    }
+    if args.no_quality:
+        return timing
     plain = {}
     for d in PLAIN_SWEEP:
         for kind, den in (("plain", lgx.DENOISER_BILATERAL), ("raw", lgx.DENOISER_NONE)):
@@ -282,9 +296,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather (step growth and lod of {dn.NOISY_FOOTPRINT} "
                     f"directions), {STEPS} steps, {LEVELS}-level Hi-Z, 4x4 interleave, BF16 U-Net denoiser pass "
                     "(3 levels, 32/64/128 channels, FP32 accumulation)",
-        "ms_per_frame": round(den_ms, 4),
-        "unet_ms": round(unet_ms, 4),
-        "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(den_passes)},
+        **timing,
         "rel_l2_vs_fp64_256": round(err, 5),
         "rel_l2_per_seed": [round(x, 5) for x in den_err],
         "noisy_rel_l2": round(float(np.mean(noisy_err)), 5),
@@ -296,12 +308,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         "plain_ms_at_equal_error": round(eq_ms, 4) if eq_ms else None,
         "speedup_at_equal_error": round(eq_ms / den_ms, 3) if eq_ms else None,
         "speedup_vs_plain32": round(plain32_ms / den_ms, 3),
-        "unet_roofline": {"bound": "mfma", "achieved": round(flops / (unet_ms * 1e-3) / 1e12, 2),
-                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                          "frac": round(flops / (unet_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
-                          "flops_per_frame": flops},
         "weights": os.path.relpath(path, ROOT),
-        "unet_tiles": [r["best"] for r in unet_tune["forward"]],
     }
 
 

@@ -322,7 +322,9 @@ int lgx_gather_batch_set_tile(lgx_gather_batch *b, int32_t tile);
 /* Logical blocks (tiles x slices) of all views together. */
 int64_t lgx_gather_batch_blocks(const lgx_gather_batch *b);
 /* workgroups: persistent workgroups (rounded up to a multiple of 8);
- * 0 = one per logical block. */
+ * 0 = one per logical block.  Workgroups take blocks from per-XCD counters
+ * that the launch zeroes first (with a kernel: capturable), so launches of
+ * ONE batch must not overlap in time (different batches may). */
 int lgx_gather_batch_launch(const lgx_gather_batch *b, int32_t workgroups, lgx_stream_t stream);
 void lgx_gather_batch_destroy(lgx_gather_batch *b);
 

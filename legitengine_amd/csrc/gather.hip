@@ -378,23 +378,36 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
 // Persistent gather over several views: the logical blocks of all views form
 // one list (view-major, first[v] = index of view v's first block, first[V] =
 // total).  The list is cut into 8 contiguous chunks, one per XCD (workgroup
-// w runs on XCD w % 8, gridDim.x is a multiple of 8), and each XCD's
-// workgroups stride through their chunk, so neighbouring tiles and the 16
-// slices of a tile share one L2.  Every workgroup ends when its stride
-// passes the chunk end.
+// w runs on XCD w % 8, gridDim.x is a multiple of 8), so neighbouring tiles
+// and the 16 slices of a tile share one L2.  Within its chunk a workgroup
+// takes the next block from the XCD's counter (zeroed before the launch):
+// blocks differ in cost (early exits, view content), and a fixed stride
+// would leave workgroups idle at the tail.  Every workgroup ends when its
+// counter passes the chunk end.
 template <class T, class A, int SAMPLER, int RFMT, int NFMT>
 __global__ __launch_bounds__(256) void gather_multi_kernel(const GatherArgs<T> *__restrict__ views,
-                                                           const uint32_t *__restrict__ first, int n_views) {
+                                                           const uint32_t *__restrict__ first, int n_views,
+                                                           uint32_t *next) {
+  __shared__ uint32_t s_item;
   const uint32_t X = 8;
   const uint32_t total = first[n_views];
-  const uint32_t xcd = blockIdx.x % X, lane_wg = blockIdx.x / X, per_xcd = gridDim.x / X;
+  const uint32_t xcd = blockIdx.x % X;
   const uint32_t chunk = (total + X - 1) / X;
   const uint32_t c0 = xcd * chunk, c1 = min(total, c0 + chunk);
   int v = 0;
-  for (uint32_t it = c0 + lane_wg; it < c1; it += per_xcd) {
-    while (v + 1 < n_views && it >= first[v + 1]) v++;  // it only grows
+  for (;;) {
+    if (threadIdx.x == 0) s_item = c0 + atomicAdd(&next[xcd], 1u);
+    __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();  // every wave has read s_item before lane 0 overwrites it
+    if (it >= c1) break;
+    while (v + 1 < n_views && it >= first[v + 1]) v++;  // a workgroup's items only grow
     gather_block<T, A, SAMPLER, RFMT, NFMT, false>(views[v], it - first[v]);
   }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------- host side
@@ -709,7 +722,7 @@ struct lgx_gather_batch {
   std::vector<lgx_image> out;
   int sampler = 0, rf = 0, nf = 0, tile = 0;
   uint32_t total = 0;
-  void *dev = nullptr;  // GatherArgs<float>[n], then uint32 first[n + 1]
+  void *dev = nullptr;  // GatherArgs<float>[n], uint32 first[n + 1], uint32 next[8] (work counters)
   ~lgx_gather_batch() {
     if (dev) (void)hipFree(dev);
   }
@@ -747,8 +760,11 @@ static hipError_t launch_multi(const lgx_gather_batch *b, unsigned grid, hipStre
   const int n = (int)b->p.size();
   const GatherArgs<float> *args = (const GatherArgs<float> *)b->dev;
   const uint32_t *first = (const uint32_t *)((const unsigned char *)b->dev + (size_t)n * sizeof(GatherArgs<float>));
+  uint32_t *next = (uint32_t *)(first + n + 1);
+  // a kernel, not a memset, so the launch can be captured into a graph
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, next, 8);
   hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT>), dim3(grid), dim3(256), 0, s, args,
-                     first, n);
+                     first, n, next);
   return hipGetLastError();
 }
 
@@ -780,7 +796,8 @@ extern "C" int lgx_gather_batch_create(const lgx_gather_params *p, const void *c
   b->sampler = p[0].sampler;
   b->rf = rf;
   b->nf = nf;
-  int rc = hip_status(hipMalloc(&b->dev, (size_t)n * sizeof(GatherArgs<float>) + ((size_t)n + 1) * sizeof(uint32_t)));
+  int rc = hip_status(
+      hipMalloc(&b->dev, (size_t)n * sizeof(GatherArgs<float>) + ((size_t)n + 1 + 8) * sizeof(uint32_t)));
   if (rc == LGX_OK) rc = gather_batch_upload(b, p[0].tile);
   if (rc != LGX_OK) {
     delete b;

@@ -256,6 +256,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
                           "frac": round(flops / (unet_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
                           "flops_per_frame": flops},
         "unet_tiles": [r["best"] for r in unet_tune["forward"]],
+        "unet_layer_ms": [round(min(r["ms"]), 4) for r in unet_tune["forward"]],
     }
     if args.no_quality:
         return timing

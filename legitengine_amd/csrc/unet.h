@@ -45,6 +45,10 @@ hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s);
 hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s);
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);
 hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12, hipStream_t s);
+// inference from the G-buffer: the BF16 input records and the head with the noisy image as residual
+hipError_t pack_gbuffer32(const lgx_image &noisy, const lgx_gbuffer &g, bf16 *x32, hipStream_t s);
+hipError_t head_fwd_img(const bf16 *d1, const lgx_image &noisy, const float *wh, const float *bh, int out_pitch,
+                        float4 *out, hipStream_t s);
 hipError_t avgpool(const bf16 *x, int n, int h, int w, int c, bf16 *y, hipStream_t s);
 hipError_t pool_bwd_add(const bf16 *dx_in, const bf16 *dpool, const bf16 *act, int n, int h, int w, int c, bf16 *dx,
                         hipStream_t s);

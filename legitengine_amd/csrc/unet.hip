@@ -257,10 +257,9 @@ class Net {
     }
   }
 
-  void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
-    const size_t P = (size_t)n * h * w;
+  // the convolutions and pools from x32 to d1
+  void body(int n, int h, int w, hipStream_t s) {
     const int h2 = h / 2, w2 = w / 2;
-    check(pack_input(x12, P, x32, s), "pack_input");
     fwd_layer(0, n, h, w, s);
     fwd_layer(1, n, h, w, s);
     check(avgpool(e1, n, h, w, 32, p1, s), "avgpool");
@@ -268,9 +267,24 @@ class Net {
     fwd_layer(3, n, h, w, s);
     check(avgpool(e2, n, h2, w2, 64, p2, s), "avgpool");
     for (int l = 4; l < 10; l++) fwd_layer(l, n, h, w, s);
+  }
+
+  void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
+    const size_t P = (size_t)n * h * w;
+    check(pack_input(x12, P, x32, s), "pack_input");
+    body(n, h, w, s);
     check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, w, out_pitch > 0 ? out_pitch : w,
                    (float4 *)out_, s),
           "head_fwd");
+  }
+
+  // inference straight from a G-buffer and the noisy image (same values as
+  // packing the records and calling forward)
+  void denoise(const lgx_image &noisy, const lgx_gbuffer &g, const lgx_image &out_, hipStream_t s) {
+    check(pack_gbuffer32(noisy, g, x32, s), "pack_gbuffer32");
+    body(1, noisy.h, noisy.w, s);
+    check(head_fwd_img(d1, noisy, params + head_w_off, params + head_b_off, out_.pitch, (float4 *)out_.ptr, s),
+          "head_fwd_img");
   }
 
   // Times every register tile of every forward (and, with backward, every
@@ -523,8 +537,7 @@ extern "C" int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_ima
   return unet_guard([&]() {
     Net &N = *u->net;
     hipStream_t s = (hipStream_t)stream;
-    N.check(lgx::unet::pack_gbuffer(*noisy, *g, N.x12_scratch, s), "pack_gbuffer");
-    N.forward(N.x12_scratch, 1, noisy->h, noisy->w, (float *)out->ptr, s, out->pitch);
+    N.denoise(*noisy, *g, *out, s);
   });
 }
 

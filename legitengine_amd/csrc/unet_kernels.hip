@@ -178,6 +178,150 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
   }
 }
 
+// The same convolution with the operands staged in LDS (output channel
+// counts up to 64).  A workgroup computes a TH = 4*WR row x TW = 16*WP pixel
+// tile for all output channels, its 4 waves stacked vertically.  Per
+// 32-channel input chunk it stages the (TH+2) x (TW+2) pixel halo tile
+// (zeros outside the image) and the chunk's weights of all 9 taps in LDS;
+// the MFMAs then read only LDS while the next chunk's global loads are in
+// flight in registers, so global latency overlaps the matrix work.  Every
+// output is accumulated in the same order as conv3x3_kernel (chunk, dx, dy),
+// so the two kernels give bit-identical results.
+template <int MB, int WR, int WP> __global__ __launch_bounds__(256) void conv3x3_lds_kernel(const ConvArgs a) {
+  constexpr int TH = 4 * WR, TW = 16 * WP, IW = TW + 2, IH = TH + 2;
+  constexpr int IN_PIECES = IH * IW * 4;     // 16-byte pieces of the input tile
+  constexpr int W_PIECES = 9 * MB * 16 * 4;  // of the chunk's weights
+  constexpr int IN_PER = (IN_PIECES + 255) / 256, W_PER = (W_PIECES + 255) / 256;
+  __shared__ __attribute__((aligned(16))) bf16 s_in[IH * IW * 32];
+  __shared__ __attribute__((aligned(16))) bf16 s_w[9 * MB * 16 * 32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = blockIdx.z, y0 = blockIdx.y * TH, x0 = blockIdx.x * TW;
+  const int col = lane & 15, kg = lane >> 4;
+  const int chunks = (a.ca + a.cb) >> 5;
+  const size_t wtap = (size_t)chunks * a.cout * 32;
+  bf16x8 pin[IN_PER], pw[W_PER];
+  auto fetch = [&](int cc) {
+    const bool from_a = cc * 32 < a.ca;
+    const bool up = from_a && a.a_up;
+    const int cs = from_a ? a.ca : a.cb;
+    const int sw = up ? (a.w >> 1) : a.w, sh = up ? (a.h >> 1) : a.h;
+    const bf16 *img = (from_a ? a.a : a.b) + (size_t)n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca);
+#pragma unroll
+    for (int i = 0; i < IN_PER; i++) {
+      const int q = tid + i * 256;
+      bf16x8 v = {};
+      if (q < IN_PIECES) {
+        const int p = q >> 2, part = q & 3;
+        const int y = y0 - 1 + p / IW, x = x0 - 1 + p % IW;
+        if (y >= 0 && y < a.h && x >= 0 && x < a.w) {
+          const uint32_t off = (uint32_t)((up ? (y >> 1) : y) * sw + (up ? (x >> 1) : x)) * (uint32_t)cs + part * 8;
+          v = *(const bf16x8 *)(img + off);
+        }
+      }
+      pin[i] = v;
+    }
+    const bf16 *wsrc = a.wt + (size_t)cc * a.cout * 32;
+#pragma unroll
+    for (int i = 0; i < W_PER; i++) {
+      const int q = tid + i * 256;
+      if (q < W_PIECES) {
+        const int tap = q / (MB * 16 * 4), rem = q % (MB * 16 * 4);
+        pw[i] = *(const bf16x8 *)(wsrc + tap * wtap + rem * 8);
+      }
+    }
+  };
+  f32x4 acc[WR][WP][MB];
+#pragma unroll
+  for (int r = 0; r < WR; r++)
+#pragma unroll
+    for (int pb = 0; pb < WP; pb++)
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) acc[r][pb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  fetch(0);
+  for (int cc = 0; cc < chunks; cc++) {
+#pragma unroll
+    for (int i = 0; i < IN_PER; i++) {
+      const int q = tid + i * 256;
+      if (q < IN_PIECES) *(bf16x8 *)(s_in + q * 8) = pin[i];
+    }
+#pragma unroll
+    for (int i = 0; i < W_PER; i++) {
+      const int q = tid + i * 256;
+      if (q < W_PIECES) *(bf16x8 *)(s_w + q * 8) = pw[i];
+    }
+    __syncthreads();
+    if (cc + 1 < chunks) fetch(cc + 1);  // in flight during the matrix work below
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++) {
+      bf16x8 bx[WR + 2][WP];
+#pragma unroll
+      for (int rr = 0; rr < WR + 2; rr++)
+#pragma unroll
+        for (int pb = 0; pb < WP; pb++)
+          bx[rr][pb] = *(const bf16x8 *)(s_in + ((wave * WR + rr) * IW + pb * 16 + col + dx) * 32 + kg * 8);
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        const int tap = dy * 3 + dx;
+        bf16x8 aw[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) aw[mb] = *(const bf16x8 *)(s_w + ((tap * MB + mb) * 16 + col) * 32 + kg * 8);
+#pragma unroll
+        for (int r = 0; r < WR; r++)
+#pragma unroll
+          for (int pb = 0; pb < WP; pb++)
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+              acc[r][pb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[mb], bx[r + dy][pb], acc[r][pb][mb], 0, 0, 0);
+      }
+    }
+    __syncthreads();  // the next chunk overwrites the tiles
+  }
+#pragma unroll
+  for (int r = 0; r < WR; r++) {
+    const int y = y0 + wave * WR + r;
+    if (y >= a.h) break;
+#pragma unroll
+    for (int pb = 0; pb < WP; pb++) {
+      const int x = x0 + pb * 16 + col;
+      if (x >= a.w) continue;
+      const size_t pix = ((size_t)n * a.h + y) * a.w + x;
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
+        const int co = mb * 16 + kg * 4;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          v[i] = acc[r][pb][mb][i];
+          if (a.bias) v[i] += a.bias[co + i];
+          if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
+        }
+        bf16 *dst;
+        const bf16 *mask;
+        int stride, cch;
+        if (co < a.split) {
+          dst = a.out0;
+          mask = a.mask0;
+          stride = a.split;
+          cch = co;
+        } else {
+          dst = a.out1;
+          mask = a.mask1;
+          stride = a.cout - a.split;
+          cch = co - a.split;
+        }
+        if (mask) {
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            if (!(bf2f(mask[pix * stride + cch + i]) > 0.f)) v[i] = 0.f;
+        }
+        typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
+        *(bf16x4 *)(dst + pix * stride + cch) = o;
+      }
+    }
+  }
+}
+
 // Weight and bias gradient of a 3x3 convolution:
 //   dW[tap][ci][co] += sum_p X[p + shift(tap)][ci] * dY[p][co],  db[co] += sum_p dY[p][co]
 // One workgroup: one image, one 32-channel group of the input, a 64-pixel
@@ -315,6 +459,29 @@ __global__ __launch_bounds__(256) void pack_gbuffer_kernel(const float4 *noisy, 
   o[9] = o[10] = o[11] = 0.f;
 }
 
+// Inference: the BF16 32-channel input straight from the G-buffer and the
+// noisy image (the values pack_gbuffer + pack_input produce, in one pass).
+template <int NFMT, int AFMT>
+__global__ __launch_bounds__(256) void pack_gbuffer32_kernel(const float4 *noisy, int np_, const float *depth, int dp,
+                                                             const void *normal, int nrmp, const void *albedo, int ap,
+                                                             int w, int h, bf16 *x32) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  const float4 c = noisy[(size_t)y * np_ + x];
+  const float4 nn = Texel<NFMT>::load(normal, (size_t)y * nrmp + x);
+  const float4 al = Texel<AFMT>::load(albedo, (size_t)y * ap + x);
+  bf16x8 v0 = {f2bf(c.x), f2bf(c.y), f2bf(c.z), f2bf(depth[(size_t)y * dp + x]), f2bf(nn.x), f2bf(nn.y),
+               f2bf(al.x), f2bf(al.y)};
+  bf16x8 v1 = {};
+  v1[0] = f2bf(al.z);
+  bf16x8 *o = (bf16x8 *)(x32 + ((size_t)y * w + x) * 32);
+  const bf16x8 z = {};
+  o[0] = v0;
+  o[1] = v1;
+  o[2] = z;
+  o[3] = z;
+}
+
 __global__ __launch_bounds__(256) void avgpool_kernel(const bf16 *x, int n, int h, int w, int c, bf16 *y) {
   // y[n][h/2][w/2][c] = mean of the 2x2 children
   const int hh = h >> 1, ww = w >> 1;
@@ -400,6 +567,25 @@ __global__ __launch_bounds__(256) void head_fwd_kernel(const bf16 *d1, const flo
     }
   }
   out[(p / w) * out_pitch + p % w] = make_float4(o[0], o[1], o[2], 1.f);
+}
+
+// The head for inference: the residual comes from the noisy image itself.
+__global__ __launch_bounds__(256) void head_fwd_img_kernel(const bf16 *d1, const float4 *noisy, int np_,
+                                                           const float *wh, const float *bh, int w, int h,
+                                                           int out_pitch, float4 *out) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= (size_t)w * h) return;
+  const int x = (int)(p % w), y = (int)(p / w);
+  const float4 c = noisy[(size_t)y * np_ + x];
+  float o[3] = {c.x + bh[0], c.y + bh[1], c.z + bh[2]};
+  for (int q = 0; q < 4; q++) {
+    const bf16x8 v = *(const bf16x8 *)(d1 + p * 32 + q * 8);
+    for (int i = 0; i < 8; i++) {
+      const float f = bf2f(v[i]);
+      for (int k = 0; k < 3; k++) o[k] += wh[k * 32 + q * 8 + i] * f;
+    }
+  }
+  out[(size_t)y * out_pitch + x] = make_float4(o[0], o[1], o[2], 1.f);
 }
 
 // L1 loss head backward: g = sign(out - target) / count per channel;
@@ -515,14 +701,14 @@ static unsigned grid_for(size_t work, size_t per_block) {
 // count MB, accumulators R * PB * MB * 4 VGPRs <= 128; entry 0 is the
 // default, the others are the autotuner's candidates.
 struct ConvTile {
-  int r, pb;
+  int r, pb, lds;  // lds: conv3x3_lds_kernel with WR = r, WP = pb
 };
-static const ConvTile kTiles1[] = {{4, 2}, {2, 4}, {4, 4}, {1, 4}};
-static const ConvTile kTiles2[] = {{4, 2}, {2, 4}, {4, 4}, {1, 4}};
-static const ConvTile kTiles4[] = {{2, 2}, {4, 2}, {2, 4}, {1, 4}};
-static const ConvTile kTiles6[] = {{2, 2}, {1, 4}, {1, 2}};
-static const ConvTile kTiles8[] = {{1, 2}, {2, 2}, {1, 4}};
-static const ConvTile kTiles12[] = {{1, 2}, {2, 1}};
+static const ConvTile kTiles1[] = {{4, 2, 0}, {2, 4, 0}, {4, 4, 0}, {1, 4, 0}};
+static const ConvTile kTiles2[] = {{4, 2, 0}, {2, 4, 0}, {4, 4, 0}, {1, 4, 0}, {2, 4, 1}, {2, 2, 1}, {1, 4, 1}};
+static const ConvTile kTiles4[] = {{2, 2, 0}, {4, 2, 0}, {2, 4, 0}, {1, 4, 0}, {2, 2, 1}, {1, 4, 1}, {1, 2, 1}};
+static const ConvTile kTiles6[] = {{2, 2, 0}, {1, 4, 0}, {1, 2, 0}};
+static const ConvTile kTiles8[] = {{1, 2, 0}, {2, 2, 0}, {1, 4, 0}};
+static const ConvTile kTiles12[] = {{1, 2, 0}, {2, 1, 0}};
 
 static const ConvTile *conv_tiles(int mb, int *count) {
 #define LGX_TILES(T) \
@@ -552,22 +738,35 @@ template <int MB, int PB, int R> static void launch_conv(const ConvArgs &a, hipS
   hipLaunchKernelGGL((conv3x3_kernel<MB, PB, R>), grid, block, 0, s, a);
 }
 
+template <int MB, int WR, int WP> static void launch_conv_lds(const ConvArgs &a, hipStream_t s) {
+  dim3 grid((a.w + 16 * WP - 1) / (16 * WP), (a.h + 4 * WR - 1) / (4 * WR), a.n), block(256);
+  hipLaunchKernelGGL((conv3x3_lds_kernel<MB, WR, WP>), grid, block, 0, s, a);
+}
+
 template <int MB> static hipError_t conv_mb(const ConvArgs &a, const ConvTile &t, hipStream_t s) {
 #define LGX_CASE(RR, PP) \
-  if (t.r == RR && t.pb == PP) { \
+  if (!t.lds && t.r == RR && t.pb == PP) { \
     launch_conv<MB, PP, RR>(a, s); \
+    return hipGetLastError(); \
+  }
+#define LGX_LDS(RR, PP) \
+  if (t.lds && t.r == RR && t.pb == PP) { \
+    launch_conv_lds<MB, RR, PP>(a, s); \
     return hipGetLastError(); \
   }
   if constexpr (MB <= 2) {
     LGX_CASE(4, 2) LGX_CASE(2, 4) LGX_CASE(4, 4) LGX_CASE(1, 4)
+    LGX_LDS(2, 4) LGX_LDS(2, 2) LGX_LDS(1, 4)
   } else if constexpr (MB <= 4) {
     LGX_CASE(2, 2) LGX_CASE(4, 2) LGX_CASE(2, 4) LGX_CASE(1, 4)
+    LGX_LDS(2, 2) LGX_LDS(1, 4) LGX_LDS(1, 2)
   } else if constexpr (MB <= 8) {
     LGX_CASE(2, 2) LGX_CASE(1, 4) LGX_CASE(1, 2)  // covers the MB = 6 and MB = 8 tables
   } else {
     LGX_CASE(1, 2) LGX_CASE(2, 1)
   }
 #undef LGX_CASE
+#undef LGX_LDS
   return hipErrorInvalidValue;
 }
 
@@ -622,6 +821,28 @@ hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12
                        g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x12);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t pack_gbuffer32(const lgx_image &noisy, const lgx_gbuffer &g, bf16 *x32, hipStream_t s) {
+  const int w = noisy.w, h = noisy.h;
+  dim3 grid((w + 63) / 64, (h + 3) / 4), block(256);
+  const float4 *np_ = (const float4 *)noisy.ptr;
+  const float *dp = (const float *)g.depth.ptr;
+  if (g.normal.format == LGX_RG16F && g.albedo.format == LGX_RGBA8)
+    hipLaunchKernelGGL((pack_gbuffer32_kernel<LGX_RG16F, LGX_RGBA8>), grid, block, 0, s, np_, noisy.pitch, dp,
+                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x32);
+  else if (g.normal.format == LGX_RG32F && g.albedo.format == LGX_RGBA32F)
+    hipLaunchKernelGGL((pack_gbuffer32_kernel<LGX_RG32F, LGX_RGBA32F>), grid, block, 0, s, np_, noisy.pitch, dp,
+                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x32);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t head_fwd_img(const bf16 *d1, const lgx_image &noisy, const float *wh, const float *bh, int out_pitch,
+                        float4 *out, hipStream_t s) {
+  const size_t npix = (size_t)noisy.w * noisy.h;
+  hipLaunchKernelGGL(head_fwd_img_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, d1,
+                     (const float4 *)noisy.ptr, noisy.pitch, wh, bh, noisy.w, noisy.h, out_pitch, out);
   return hipGetLastError();
 }
 hipError_t avgpool(const bf16 *x, int n, int h, int w, int c, bf16 *y, hipStream_t s) {

@@ -20,6 +20,8 @@
 // consecutive output channels of one pixel (one 8-byte store).
 #include <math.h>
 
+#include <algorithm>
+
 #include "lgx_internal.h"
 #include "unet.h"
 
@@ -178,41 +180,63 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
   }
 }
 
-// The same convolution with the operands staged in LDS (output channel
-// counts up to 64).  A workgroup computes a TH = 4*WR row x TW = 16*WP pixel
-// tile for all output channels, its 4 waves stacked vertically.  Per
-// 32-channel input chunk it stages the (TH+2) x (TW+2) pixel halo tile
-// (zeros outside the image) and the chunk's weights of all 9 taps in LDS;
-// the MFMAs then read only LDS while the next chunk's global loads are in
-// flight in registers, so global latency overlaps the matrix work.  Every
-// output is accumulated in the same order as conv3x3_kernel (chunk, dx, dy),
-// so the two kernels give bit-identical results.
-template <int MB, int WR, int WP> __global__ __launch_bounds__(256) void conv3x3_lds_kernel(const ConvArgs a) {
+// The same convolution with the operands staged in LDS.  A workgroup
+// computes a TH = 4*WR row x TW = 16*WP pixel tile for MB*16 output channels
+// (a wide layer's channels are split over `splits` tiles), its 4 waves
+// stacked vertically.  Per 32-channel input chunk it stages the
+// (TH+2) x (TW+2) pixel halo tile (zeros outside the image) and the chunk's
+// weights of all 9 taps in LDS; the MFMAs read only LDS while the global
+// loads of the next (tile, chunk) are in flight in registers, so global
+// latency overlaps the matrix work also across tiles: a workgroup walks
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ...  Every output is
+// accumulated in the same order as conv3x3_kernel (chunk, dx, dy), so the
+// two kernels give bit-identical results.
+template <int MB, int WR, int WP>
+__global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x, int tiles_y, int splits) {
   constexpr int TH = 4 * WR, TW = 16 * WP, IW = TW + 2, IH = TH + 2;
   constexpr int IN_PIECES = IH * IW * 4;     // 16-byte pieces of the input tile
   constexpr int W_PIECES = 9 * MB * 16 * 4;  // of the chunk's weights
   constexpr int IN_PER = (IN_PIECES + 255) / 256, W_PER = (W_PIECES + 255) / 256;
-  __shared__ __attribute__((aligned(16))) bf16 s_in[IH * IW * 32];
-  __shared__ __attribute__((aligned(16))) bf16 s_w[9 * MB * 16 * 32];
+  // LDS layout [channel group kg][pixel or weight row][8 channels]: the 8
+  // consecutive lanes an LDS access serves together (same kg, consecutive
+  // pixels / rows) read 128 contiguous bytes, free of bank conflicts; the
+  // weight planes are padded by 64 bytes so the stores spread over banks too.
+  constexpr int PLANE_IN = IH * IW * 8, PLANE_W = 9 * MB * 16 * 8 + 32;
+  __shared__ __attribute__((aligned(16))) bf16 s_in[4 * PLANE_IN];
+  __shared__ __attribute__((aligned(16))) bf16 s_w[4 * PLANE_W];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = blockIdx.z, y0 = blockIdx.y * TH, x0 = blockIdx.x * TW;
   const int col = lane & 15, kg = lane >> 4;
   const int chunks = (a.ca + a.cb) >> 5;
   const size_t wtap = (size_t)chunks * a.cout * 32;
+  const int T = tiles_x * tiles_y * a.n * splits;
+  struct Tile {
+    int n, y0, x0, co0;
+  };
+  // consecutive tile indices: the channel splits of one pixel tile, then x, y, image
+  auto tile_of = [&](int t) {
+    Tile r;
+    r.co0 = (t % splits) * MB * 16;
+    int u = t / splits;
+    r.x0 = (u % tiles_x) * TW;
+    u /= tiles_x;
+    r.y0 = (u % tiles_y) * TH;
+    r.n = u / tiles_y;
+    return r;
+  };
   bf16x8 pin[IN_PER], pw[W_PER];
-  auto fetch = [&](int cc) {
+  auto fetch = [&](const Tile &tl, int cc) {
     const bool from_a = cc * 32 < a.ca;
     const bool up = from_a && a.a_up;
     const int cs = from_a ? a.ca : a.cb;
     const int sw = up ? (a.w >> 1) : a.w, sh = up ? (a.h >> 1) : a.h;
-    const bf16 *img = (from_a ? a.a : a.b) + (size_t)n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca);
+    const bf16 *img = (from_a ? a.a : a.b) + (size_t)tl.n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca);
 #pragma unroll
     for (int i = 0; i < IN_PER; i++) {
       const int q = tid + i * 256;
       bf16x8 v = {};
       if (q < IN_PIECES) {
         const int p = q >> 2, part = q & 3;
-        const int y = y0 - 1 + p / IW, x = x0 - 1 + p % IW;
+        const int y = tl.y0 - 1 + p / IW, x = tl.x0 - 1 + p % IW;
         if (y >= 0 && y < a.h && x >= 0 && x < a.w) {
           const uint32_t off = (uint32_t)((up ? (y >> 1) : y) * sw + (up ? (x >> 1) : x)) * (uint32_t)cs + part * 8;
           v = *(const bf16x8 *)(img + off);
@@ -220,7 +244,7 @@ template <int MB, int WR, int WP> __global__ __launch_bounds__(256) void conv3x3
       }
       pin[i] = v;
     }
-    const bf16 *wsrc = a.wt + (size_t)cc * a.cout * 32;
+    const bf16 *wsrc = a.wt + ((size_t)cc * a.cout + tl.co0) * 32;
 #pragma unroll
     for (int i = 0; i < W_PER; i++) {
       const int q = tid + i * 256;
@@ -230,95 +254,109 @@ template <int MB, int WR, int WP> __global__ __launch_bounds__(256) void conv3x3
       }
     }
   };
-  f32x4 acc[WR][WP][MB];
+  int t = blockIdx.x;
+  if (t >= T) return;  // the whole workgroup, before any barrier
+  Tile cur = tile_of(t);
+  fetch(cur, 0);
+  for (;;) {
+    f32x4 acc[WR][WP][MB];
 #pragma unroll
-  for (int r = 0; r < WR; r++)
+    for (int r = 0; r < WR; r++)
 #pragma unroll
-    for (int pb = 0; pb < WP; pb++)
+      for (int pb = 0; pb < WP; pb++)
 #pragma unroll
-      for (int mb = 0; mb < MB; mb++) acc[r][pb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  fetch(0);
-  for (int cc = 0; cc < chunks; cc++) {
+        for (int mb = 0; mb < MB; mb++) acc[r][pb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int cc = 0; cc < chunks; cc++) {
 #pragma unroll
-    for (int i = 0; i < IN_PER; i++) {
-      const int q = tid + i * 256;
-      if (q < IN_PIECES) *(bf16x8 *)(s_in + q * 8) = pin[i];
-    }
+      for (int i = 0; i < IN_PER; i++) {
+        const int q = tid + i * 256;
+        if (q < IN_PIECES) *(bf16x8 *)(s_in + (q & 3) * PLANE_IN + (q >> 2) * 8) = pin[i];
+      }
 #pragma unroll
-    for (int i = 0; i < W_PER; i++) {
-      const int q = tid + i * 256;
-      if (q < W_PIECES) *(bf16x8 *)(s_w + q * 8) = pw[i];
-    }
-    __syncthreads();
-    if (cc + 1 < chunks) fetch(cc + 1);  // in flight during the matrix work below
+      for (int i = 0; i < W_PER; i++) {
+        const int q = tid + i * 256;
+        if (q < W_PIECES) *(bf16x8 *)(s_w + (q & 3) * PLANE_W + (q >> 2) * 8) = pw[i];
+      }
+      __syncthreads();
+      // in flight during the matrix work below: the next chunk, or the next tile's first
+      if (cc + 1 < chunks)
+        fetch(cur, cc + 1);
+      else if (t + (int)gridDim.x < T)
+        fetch(tile_of(t + (int)gridDim.x), 0);
 #pragma unroll
-    for (int dx = 0; dx < 3; dx++) {
-      bf16x8 bx[WR + 2][WP];
+      for (int dx = 0; dx < 3; dx++) {
+        bf16x8 bx[WR + 2][WP];
 #pragma unroll
-      for (int rr = 0; rr < WR + 2; rr++)
-#pragma unroll
-        for (int pb = 0; pb < WP; pb++)
-          bx[rr][pb] = *(const bf16x8 *)(s_in + ((wave * WR + rr) * IW + pb * 16 + col + dx) * 32 + kg * 8);
-#pragma unroll
-      for (int dy = 0; dy < 3; dy++) {
-        const int tap = dy * 3 + dx;
-        bf16x8 aw[MB];
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++) aw[mb] = *(const bf16x8 *)(s_w + ((tap * MB + mb) * 16 + col) * 32 + kg * 8);
-#pragma unroll
-        for (int r = 0; r < WR; r++)
+        for (int rr = 0; rr < WR + 2; rr++)
 #pragma unroll
           for (int pb = 0; pb < WP; pb++)
+            bx[rr][pb] = *(const bf16x8 *)(s_in + kg * PLANE_IN + ((wave * WR + rr) * IW + pb * 16 + col + dx) * 8);
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++)
-              acc[r][pb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[mb], bx[r + dy][pb], acc[r][pb][mb], 0, 0, 0);
+        for (int dy = 0; dy < 3; dy++) {
+          const int tap = dy * 3 + dx;
+          bf16x8 aw[MB];
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++)
+            aw[mb] = *(const bf16x8 *)(s_w + kg * PLANE_W + ((tap * MB + mb) * 16 + col) * 8);
+#pragma unroll
+          for (int r = 0; r < WR; r++)
+#pragma unroll
+            for (int pb = 0; pb < WP; pb++)
+#pragma unroll
+              for (int mb = 0; mb < MB; mb++)
+                acc[r][pb][mb] =
+                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[mb], bx[r + dy][pb], acc[r][pb][mb], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // the next stash overwrites the tiles
+    }
+#pragma unroll
+    for (int r = 0; r < WR; r++) {
+      const int y = cur.y0 + wave * WR + r;
+      if (y >= a.h) break;
+#pragma unroll
+      for (int pb = 0; pb < WP; pb++) {
+        const int x = cur.x0 + pb * 16 + col;
+        if (x >= a.w) continue;
+        const size_t pix = ((size_t)cur.n * a.h + y) * a.w + x;
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+          const int co = cur.co0 + mb * 16 + kg * 4;
+          float v[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            v[i] = acc[r][pb][mb][i];
+            if (a.bias) v[i] += a.bias[co + i];
+            if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
+          }
+          bf16 *dst;
+          const bf16 *mask;
+          int stride, cch;
+          if (co < a.split) {
+            dst = a.out0;
+            mask = a.mask0;
+            stride = a.split;
+            cch = co;
+          } else {
+            dst = a.out1;
+            mask = a.mask1;
+            stride = a.cout - a.split;
+            cch = co - a.split;
+          }
+          if (mask) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              if (!(bf2f(mask[pix * stride + cch + i]) > 0.f)) v[i] = 0.f;
+          }
+          typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+          bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
+          *(bf16x4 *)(dst + pix * stride + cch) = o;
+        }
       }
     }
-    __syncthreads();  // the next chunk overwrites the tiles
-  }
-#pragma unroll
-  for (int r = 0; r < WR; r++) {
-    const int y = y0 + wave * WR + r;
-    if (y >= a.h) break;
-#pragma unroll
-    for (int pb = 0; pb < WP; pb++) {
-      const int x = x0 + pb * 16 + col;
-      if (x >= a.w) continue;
-      const size_t pix = ((size_t)n * a.h + y) * a.w + x;
-#pragma unroll
-      for (int mb = 0; mb < MB; mb++) {
-        const int co = mb * 16 + kg * 4;
-        float v[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          v[i] = acc[r][pb][mb][i];
-          if (a.bias) v[i] += a.bias[co + i];
-          if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
-        }
-        bf16 *dst;
-        const bf16 *mask;
-        int stride, cch;
-        if (co < a.split) {
-          dst = a.out0;
-          mask = a.mask0;
-          stride = a.split;
-          cch = co;
-        } else {
-          dst = a.out1;
-          mask = a.mask1;
-          stride = a.cout - a.split;
-          cch = co - a.split;
-        }
-        if (mask) {
-#pragma unroll
-          for (int i = 0; i < 4; i++)
-            if (!(bf2f(mask[pix * stride + cch + i]) > 0.f)) v[i] = 0.f;
-        }
-        typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
-        bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
-        *(bf16x4 *)(dst + pix * stride + cch) = o;
-      }
-    }
+    t += gridDim.x;
+    if (t >= T) break;
+    cur = tile_of(t);
   }
 }
 
@@ -702,13 +740,21 @@ static unsigned grid_for(size_t work, size_t per_block) {
 // default, the others are the autotuner's candidates.
 struct ConvTile {
   int r, pb, lds;  // lds: conv3x3_lds_kernel with WR = r, WP = pb
+  int mbw;         // lds: output-channel blocks per workgroup (cout / (16 * mbw) splits)
+  int per_cu;      // lds: workgroups per CU walking the tiles, 0 = one workgroup per tile
 };
-static const ConvTile kTiles1[] = {{4, 2, 0}, {2, 4, 0}, {4, 4, 0}, {1, 4, 0}};
-static const ConvTile kTiles2[] = {{4, 2, 0}, {2, 4, 0}, {4, 4, 0}, {1, 4, 0}, {2, 4, 1}, {2, 2, 1}, {1, 4, 1}};
-static const ConvTile kTiles4[] = {{2, 2, 0}, {4, 2, 0}, {2, 4, 0}, {1, 4, 0}, {2, 2, 1}, {1, 4, 1}, {1, 2, 1}};
-static const ConvTile kTiles6[] = {{2, 2, 0}, {1, 4, 0}, {1, 2, 0}};
-static const ConvTile kTiles8[] = {{1, 2, 0}, {2, 2, 0}, {1, 4, 0}};
-static const ConvTile kTiles12[] = {{1, 2, 0}, {2, 1, 0}};
+static const ConvTile kTiles1[] = {{4, 2, 0, 1, 0}, {2, 4, 0, 1, 0}, {4, 4, 0, 1, 0}, {1, 4, 0, 1, 0}};
+static const ConvTile kTiles2[] = {{4, 2, 0, 2, 0}, {2, 4, 0, 2, 0}, {4, 4, 0, 2, 0}, {1, 4, 0, 2, 0},
+                                   {2, 4, 1, 2, 0}, {2, 2, 1, 2, 0}, {1, 4, 1, 2, 0}, {2, 4, 1, 2, 2},
+                                   {2, 2, 1, 2, 3}, {1, 4, 1, 2, 3}};
+static const ConvTile kTiles4[] = {{2, 2, 0, 4, 0}, {4, 2, 0, 4, 0}, {2, 4, 0, 4, 0}, {1, 4, 0, 4, 0},
+                                   {2, 2, 1, 4, 0}, {1, 4, 1, 4, 0}, {1, 2, 1, 4, 0}, {2, 2, 1, 4, 2},
+                                   {1, 2, 1, 4, 3}, {2, 4, 1, 2, 0}, {2, 2, 1, 2, 3}};
+static const ConvTile kTiles6[] = {{2, 2, 0, 6, 0}, {1, 4, 0, 6, 0}, {1, 2, 0, 6, 0}, {2, 4, 1, 2, 0},
+                                   {2, 2, 1, 2, 3}};
+static const ConvTile kTiles8[] = {{1, 2, 0, 8, 0}, {2, 2, 0, 8, 0}, {1, 4, 0, 8, 0}, {2, 2, 1, 4, 0},
+                                   {1, 2, 1, 4, 3}, {2, 2, 1, 4, 2}};
+static const ConvTile kTiles12[] = {{1, 2, 0, 12, 0}, {2, 1, 0, 12, 0}, {2, 2, 1, 4, 0}, {1, 2, 1, 4, 3}};
 
 static const ConvTile *conv_tiles(int mb, int *count) {
 #define LGX_TILES(T) \
@@ -738,9 +784,24 @@ template <int MB, int PB, int R> static void launch_conv(const ConvArgs &a, hipS
   hipLaunchKernelGGL((conv3x3_kernel<MB, PB, R>), grid, block, 0, s, a);
 }
 
-template <int MB, int WR, int WP> static void launch_conv_lds(const ConvArgs &a, hipStream_t s) {
-  dim3 grid((a.w + 16 * WP - 1) / (16 * WP), (a.h + 4 * WR - 1) / (4 * WR), a.n), block(256);
-  hipLaunchKernelGGL((conv3x3_lds_kernel<MB, WR, WP>), grid, block, 0, s, a);
+static int cu_count() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) ==
+                                                hipSuccess && v > 0)
+      cus = v;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
+template <int MBW, int WR, int WP> static void launch_conv_lds(const ConvArgs &a, int per_cu, hipStream_t s) {
+  const int tx = (a.w + 16 * WP - 1) / (16 * WP), ty = (a.h + 4 * WR - 1) / (4 * WR), splits = a.cout / (16 * MBW);
+  const long tiles = (long)tx * ty * a.n * splits;
+  long grid = per_cu > 0 ? std::min(tiles, (long)per_cu * cu_count()) : tiles;
+  hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty, splits);
 }
 
 template <int MB> static hipError_t conv_mb(const ConvArgs &a, const ConvTile &t, hipStream_t s) {
@@ -749,21 +810,25 @@ template <int MB> static hipError_t conv_mb(const ConvArgs &a, const ConvTile &t
     launch_conv<MB, PP, RR>(a, s); \
     return hipGetLastError(); \
   }
-#define LGX_LDS(RR, PP) \
-  if (t.lds && t.r == RR && t.pb == PP) { \
-    launch_conv_lds<MB, RR, PP>(a, s); \
+#define LGX_LDS(MBW, RR, PP) \
+  if (t.lds && t.mbw == MBW && t.r == RR && t.pb == PP) { \
+    launch_conv_lds<MBW, RR, PP>(a, t.per_cu, s); \
     return hipGetLastError(); \
   }
   if constexpr (MB <= 2) {
     LGX_CASE(4, 2) LGX_CASE(2, 4) LGX_CASE(4, 4) LGX_CASE(1, 4)
-    LGX_LDS(2, 4) LGX_LDS(2, 2) LGX_LDS(1, 4)
   } else if constexpr (MB <= 4) {
     LGX_CASE(2, 2) LGX_CASE(4, 2) LGX_CASE(2, 4) LGX_CASE(1, 4)
-    LGX_LDS(2, 2) LGX_LDS(1, 4) LGX_LDS(1, 2)
   } else if constexpr (MB <= 8) {
     LGX_CASE(2, 2) LGX_CASE(1, 4) LGX_CASE(1, 2)  // covers the MB = 6 and MB = 8 tables
   } else {
     LGX_CASE(1, 2) LGX_CASE(2, 1)
+  }
+  if constexpr (MB % 2 == 0) {
+    LGX_LDS(2, 2, 4) LGX_LDS(2, 2, 2) LGX_LDS(2, 1, 4)
+  }
+  if constexpr (MB % 4 == 0) {
+    LGX_LDS(4, 2, 2) LGX_LDS(4, 1, 4) LGX_LDS(4, 1, 2)
   }
 #undef LGX_CASE
 #undef LGX_LDS

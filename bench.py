@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--cpu-sample-px", type=int, default=96 * 96)
     ap.add_argument("--no-denoised", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--no-quality", action="store_true",
                     help="time the denoised path only (no error sweep against the FP64 target)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -144,6 +145,10 @@ def main():
         result["denoised"] = denoised_leg(args, g, view, proj, sampler, ms_per_step, seeds)
         log(f"denoised path: {result['denoised']['ms_per_frame']:.3f} ms/frame, "
             f"rel-L2 {result['denoised'].get('rel_l2_vs_fp64_256')}")
+
+    if not args.no_train:
+        result["training"] = train_leg(args)
+        log(f"training: {result['training']['steps_per_s']} steps/s")
 
     if not args.no_batch:
         result["uneven_batch"] = batch_leg(args, sampler)
@@ -240,8 +245,8 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     net = dev.UNet(1, H, W)
     net.load(path)
     unet_tune = net.autotune(1, H, W)
-    rden = dev.Renderer(W, H, DEN_DIRS, STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_UNET, kind="mixed",
-                        profile=True, footprint_dirs=dn.NOISY_FOOTPRINT)
+    rden = dev.Renderer(W, H, DEN_DIRS, dn.NOISY_STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_UNET,
+                        kind="mixed", profile=True, footprint_dirs=dn.NOISY_FOOTPRINT)
     rden.attach_unet(net)
     den_ms, den_passes = time_renderer(rden, g, view, proj, args.steps, args.warmup)
     log(f"denoised renderer timed: {den_ms:.3f} ms/frame")
@@ -295,8 +300,8 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         e["renderer"].close()
     return {
         "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather (step growth and lod of {dn.NOISY_FOOTPRINT} "
-                    f"directions), {STEPS} steps, {LEVELS}-level Hi-Z, 4x4 interleave, BF16 U-Net denoiser pass "
-                    "(3 levels, 32/64/128 channels, FP32 accumulation)",
+                    f"directions, reference step rule without cap), {LEVELS}-level Hi-Z, 4x4 interleave, BF16 U-Net "
+                    "denoiser pass (3 levels, 32/64/128 channels, FP32 accumulation)",
         **timing,
         "rel_l2_vs_fp64_256": round(err, 5),
         "rel_l2_per_seed": [round(x, 5) for x in den_err],
@@ -311,6 +316,38 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         "speedup_vs_plain32": round(plain32_ms / den_ms, 3),
         "weights": os.path.relpath(path, ROOT),
     }
+
+
+def train_leg(args, views=64, steps=60, warmup=5):
+    """Training throughput on the training configuration: batch 16 of 256x256
+    crops, forward + backward + Adam on the project's kernels (pairs from a
+    small set of seeded views; the throughput does not depend on the data)."""
+    import torch
+    from legitengine_amd import device as dev
+    from legitengine_amd import denoiser as dn
+    X, T = dn.build_dataset(views // 4, 4, log=lambda m: None)
+    X = X[dn.NOISY_FOOTPRINT]
+    net = dev.UNet(16, dn.CROP, dn.CROP, seed=0)
+    net.autotune(16, dn.CROP, dn.CROP, backward=True)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    batches = [(X.index_select(0, idx).contiguous(), T.index_select(0, idx).contiguous())
+               for idx in (torch.randint(0, X.shape[0], (16,), device="cuda", generator=gen) for _ in range(8))]
+    for i in range(warmup):
+        net.train_step(*batches[i % 8], 1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        net.train_step(*batches[i % 8], 1e-3)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    flops = 3 * dn.unet_flops_per_pixel() * 16 * dn.CROP * dn.CROP  # forward + input and weight gradients
+    return {"workload": "U-Net training step: batch 16 of 256x256 crops, BF16 MFMA forward + backward, FP32 Adam",
+            "steps_per_s": round(steps / secs, 2), "ms_per_step": round(secs * 1e3 / steps, 3),
+            "mfma_roofline": {"achieved": round(flops * steps / secs / 1e12, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": round(flops * steps / secs / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
+                              "flops_per_step": flops},
+            "gradient_check": "FP64 central finite differences on a 16x16 crop: tests/test_unet_oracle.py"}
 
 
 def batch_resolution(seed):
@@ -340,7 +377,7 @@ def batch_leg(args, sampler):
     path = args.weights or dn.WEIGHTS
     net.load(path)
     net.autotune(1, 1080, 1920)  # register tiles at the most common extent
-    b = dev.Batch(views, dirs=DEN_DIRS, max_steps=STEPS, levels=LEVELS, sampler=sampler, grid=GRID,
+    b = dev.Batch(views, dirs=DEN_DIRS, max_steps=dn.NOISY_STEPS, levels=LEVELS, sampler=sampler, grid=GRID,
                   footprint_dirs=dn.NOISY_FOOTPRINT, net=net)
     log(f"batch of {len(views)} views created")
     tune = b.autotune()

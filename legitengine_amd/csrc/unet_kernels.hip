@@ -223,35 +223,36 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
     r.n = u / tiles_y;
     return r;
   };
+  // Loads are unconditional (clamped to an in-image pixel / a valid piece)
+  // and padding is applied when staging: a conditional load would make the
+  // compiler merge values right after it and wait for the load there,
+  // serialising the prefetch with the matrix work.
   bf16x8 pin[IN_PER], pw[W_PER];
+  uint32_t valid = 0;  // bit i: piece i of pin lies in the image
   auto fetch = [&](const Tile &tl, int cc) {
     const bool from_a = cc * 32 < a.ca;
     const bool up = from_a && a.a_up;
     const int cs = from_a ? a.ca : a.cb;
     const int sw = up ? (a.w >> 1) : a.w, sh = up ? (a.h >> 1) : a.h;
     const bf16 *img = (from_a ? a.a : a.b) + (size_t)tl.n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca);
+    valid = 0;
 #pragma unroll
     for (int i = 0; i < IN_PER; i++) {
-      const int q = tid + i * 256;
-      bf16x8 v = {};
-      if (q < IN_PIECES) {
-        const int p = q >> 2, part = q & 3;
-        const int y = tl.y0 - 1 + p / IW, x = tl.x0 - 1 + p % IW;
-        if (y >= 0 && y < a.h && x >= 0 && x < a.w) {
-          const uint32_t off = (uint32_t)((up ? (y >> 1) : y) * sw + (up ? (x >> 1) : x)) * (uint32_t)cs + part * 8;
-          v = *(const bf16x8 *)(img + off);
-        }
-      }
-      pin[i] = v;
+      const int q = min(tid + i * 256, IN_PIECES - 1);
+      const int p = q >> 2, part = q & 3;
+      const int y = tl.y0 - 1 + p / IW, x = tl.x0 - 1 + p % IW;
+      const bool ok = y >= 0 && y < a.h && x >= 0 && x < a.w;
+      const int yc = min(max(y, 0), a.h - 1), xc = min(max(x, 0), a.w - 1);
+      const uint32_t off = (uint32_t)((up ? (yc >> 1) : yc) * sw + (up ? (xc >> 1) : xc)) * (uint32_t)cs + part * 8;
+      pin[i] = *(const bf16x8 *)(img + off);
+      valid |= (uint32_t)ok << i;
     }
     const bf16 *wsrc = a.wt + ((size_t)cc * a.cout + tl.co0) * 32;
 #pragma unroll
     for (int i = 0; i < W_PER; i++) {
-      const int q = tid + i * 256;
-      if (q < W_PIECES) {
-        const int tap = q / (MB * 16 * 4), rem = q % (MB * 16 * 4);
-        pw[i] = *(const bf16x8 *)(wsrc + tap * wtap + rem * 8);
-      }
+      const int q = min(tid + i * 256, W_PIECES - 1);
+      const int tap = q / (MB * 16 * 4), rem = q % (MB * 16 * 4);
+      pw[i] = *(const bf16x8 *)(wsrc + tap * wtap + rem * 8);
     }
   };
   int t = blockIdx.x;
@@ -270,7 +271,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
 #pragma unroll
       for (int i = 0; i < IN_PER; i++) {
         const int q = tid + i * 256;
-        if (q < IN_PIECES) *(bf16x8 *)(s_in + (q & 3) * PLANE_IN + (q >> 2) * 8) = pin[i];
+        const bf16x8 z = {};
+        if (q < IN_PIECES) *(bf16x8 *)(s_in + (q & 3) * PLANE_IN + (q >> 2) * 8) = ((valid >> i) & 1u) ? pin[i] : z;
       }
 #pragma unroll
       for (int i = 0; i < W_PER; i++) {

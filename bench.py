@@ -88,12 +88,15 @@ def main():
         traffic_probe(g, view, proj, sampler)
         return
     # The drop-in path: SsvgiRenderer passes declared on the C++ RenderGraph,
-    # timed per pass with hipEvents on the frame's stream.
+    # timed per pass with hipEvents on the frame's stream; the gather's launch
+    # shape comes from the autotuner (all shapes give identical results).
+    pyr = dev.PyramidDev(g, LEVELS, sampler).build()
+    plain_tile, plain_tiles = dev.autotune_gather(g, pyr, view, proj, DIRS, STEPS, sampler)
     renderer = dev.Renderer(W, H, DIRS, STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_BILATERAL,
-                            denoise_passes=BILAT_PASSES, denoise_radius=BILAT_RADIUS, kind="mixed", profile=True)
+                            denoise_passes=BILAT_PASSES, denoise_radius=BILAT_RADIUS, kind="mixed", profile=True,
+                            tile=plain_tile)
 
     # step counts (outside the timed region) for the algorithmic byte model
-    pyr = dev.PyramidDev(g, LEVELS, sampler).build()
     counts = torch.zeros((H, W), dtype=torch.int32, device="cuda")
     dev.Gather(W, H, DIRS, STEPS, sampler)(view, proj, g, pyr, step_counts=counts)
     torch.cuda.synchronize()
@@ -136,6 +139,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "gather_kernel (IndirectLightPass)", "kernel_ms": round(gather_ms, 4),
+                     "tile": plain_tile, "tile_ms": {k: round(v, 4) for k, v in plain_tiles.items()},
                      "algorithmic_bytes": gather_bytes, "steps_executed": total_steps},
         "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(passes)},
     }
@@ -245,8 +249,10 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     net = dev.UNet(1, H, W)
     net.load(path)
     unet_tune = net.autotune(1, H, W)
+    pyr0 = dev.PyramidDev(g, LEVELS, sampler).build()
+    den_tile, _ = dev.autotune_gather(g, pyr0, view, proj, DEN_DIRS, dn.NOISY_STEPS, sampler, dn.NOISY_FOOTPRINT)
     rden = dev.Renderer(W, H, DEN_DIRS, dn.NOISY_STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_UNET,
-                        kind="mixed", profile=True, footprint_dirs=dn.NOISY_FOOTPRINT)
+                        kind="mixed", profile=True, footprint_dirs=dn.NOISY_FOOTPRINT, tile=den_tile)
     rden.attach_unet(net)
     den_ms, den_passes = time_renderer(rden, g, view, proj, args.steps, args.warmup)
     log(f"denoised renderer timed: {den_ms:.3f} ms/frame")
@@ -262,6 +268,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
                           "flops_per_frame": flops},
         "unet_tiles": [r["best"] for r in unet_tune["forward"]],
         "unet_layer_ms": [round(min(r["ms"]), 4) for r in unet_tune["forward"]],
+        "gather_tile": den_tile,
     }
     if args.no_quality:
         return timing

@@ -52,6 +52,7 @@ template <class T> struct GatherArgs {
   int out_pitch, out_slices;
   int *step_counts;
   int tile_w, tile_h, tiles_x, n_blocks;
+  int occ;  // host side: which register allocation to launch
   int sx0, sy0, lcells_x, lcells_y;  // slice cells covered by the launch (region of interest)
   int rx0, ry0, rx1, ry1;            // pixels gathered: [rx0, rx1) x [ry0, ry1)
 };
@@ -370,8 +371,10 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
   if (COUNT) a.step_counts[(size_t)y * a.w + x] = step_total;
 }
 
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT>
-__global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
+// OCC: minimum waves per SIMD the register allocation must allow (1 = the
+// compiler's choice); the launch shapes offer 1 and 8.
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, int OCC>
+__global__ __launch_bounds__(256, OCC) void gather_kernel(const GatherArgs<T> a) {
   gather_block<T, A, SAMPLER, RFMT, NFMT, COUNT>(a, xcd_remap(blockIdx.x, (uint32_t)a.n_blocks));
 }
 
@@ -384,8 +387,8 @@ __global__ __launch_bounds__(256) void gather_kernel(const GatherArgs<T> a) {
 // blocks differ in cost (early exits, view content), and a fixed stride
 // would leave workgroups idle at the tail.  Every workgroup ends when its
 // counter passes the chunk end.
-template <class T, class A, int SAMPLER, int RFMT, int NFMT>
-__global__ __launch_bounds__(256) void gather_multi_kernel(const GatherArgs<T> *__restrict__ views,
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, int OCC>
+__global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs<T> *__restrict__ views,
                                                            const uint32_t *__restrict__ first, int n_views,
                                                            uint32_t *next) {
   __shared__ uint32_t s_item;
@@ -548,9 +551,14 @@ template <class T, class A, int SAMPLER, int RFMT, int NFMT>
 static hipError_t launch_gather(const GatherArgs<T> &a, bool count, hipStream_t s) {
   dim3 grid((unsigned)a.n_blocks), block(256);
   if (count)
-    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, true>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, true, 1>), grid, block, 0, s, a);
+  else if (a.occ == 8) {
+    if constexpr (sizeof(T) == 4 && sizeof(A) == 4)
+      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 8>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1>), grid, block, 0, s, a);
+  } else
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -564,9 +572,12 @@ static hipError_t dispatch_fmt(const GatherArgs<T> &a, int rfmt, int nfmt, bool 
 }
 
 // Launch shapes offered to the autotuner: tile_w x tile_h slice pixels per
-// 256-thread workgroup.
-static const int kTiles[][2] = {{16, 16}, {8, 32}, {32, 8}, {64, 4}};
-static const int kNumTiles = 4;
+// 256-thread workgroup, and the occupancy the registers are allocated for
+// (1: the compiler's choice, 79 VGPRs, 6 waves per SIMD; 8: 64 VGPRs with a
+// few spilled set-up values).  Every shape computes the same arithmetic.
+static const int kTiles[][3] = {{16, 16, 1}, {8, 32, 1}, {32, 8, 1}, {64, 4, 1},
+                                {16, 16, 8}, {8, 32, 8}, {32, 8, 8}, {64, 4, 8}};
+static const int kNumTiles = 8;
 
 }  // namespace lgx
 
@@ -636,6 +647,7 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
   if (tile < 0 || tile >= kNumTiles) return LGX_EUNSUPPORTED;
   a.tile_w = kTiles[tile][0];
   a.tile_h = kTiles[tile][1];
+  a.occ = kTiles[tile][2];
   a.rx0 = 0;
   a.ry0 = 0;
   a.rx1 = p->width;
@@ -757,14 +769,19 @@ static int gather_batch_upload(lgx_gather_batch *b, int tile) {
 
 template <int SAMPLER, int RFMT, int NFMT>
 static hipError_t launch_multi(const lgx_gather_batch *b, unsigned grid, hipStream_t s) {
+  const bool occ8 = kTiles[b->tile][2] == 8;
   const int n = (int)b->p.size();
   const GatherArgs<float> *args = (const GatherArgs<float> *)b->dev;
   const uint32_t *first = (const uint32_t *)((const unsigned char *)b->dev + (size_t)n * sizeof(GatherArgs<float>));
   uint32_t *next = (uint32_t *)(first + n + 1);
   // a kernel, not a memset, so the launch can be captured into a graph
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, next, 8);
-  hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT>), dim3(grid), dim3(256), 0, s, args,
-                     first, n, next);
+  if (occ8)
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8>), dim3(grid), dim3(256), 0, s, args,
+                       first, n, next);
+  else
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1>), dim3(grid), dim3(256), 0, s, args,
+                       first, n, next);
   return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ W, H = 1920, 1080
 LEVELS = 6
 CROP = 256
 NOISY_DIRS, TARGET_DIRS, PLAIN_DIRS, STEPS = 4, 256, 32, 64
-NOISY_FOOTPRINT = 128  # the input marches with the step growth and lod of 128 directions
+NOISY_FOOTPRINT = 256  # the input marches with the target's step growth and lod (256 directions)
 NOISY_STEPS = 0  # and the reference step rule without a cap (the far field matters most)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_1080p.safetensors")
 # (logical cin, cout, resolution divisor) of the ten 3x3 convolutions; the

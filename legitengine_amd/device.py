@@ -160,6 +160,25 @@ class Gather:
         return out
 
 
+def autotune_gather(gb, pyr, view, proj, dirs, max_steps, sampler=lgx.SAMPLER_HIZ_POINT, footprint_dirs=0, reps=3):
+    """Times every gather launch shape on this frame (hipEvents on the current
+    stream) and returns (fastest tile, {tile: ms}); all shapes give
+    bit-identical results."""
+    out = DImage(gb.w, gb.h, lgx.RGBA32F)
+    times = {}
+    for t in range(lgx.lib().lgx_gather_tiles()):
+        g = Gather(gb.w, gb.h, dirs, max_steps, sampler, tile=t, footprint_dirs=footprint_dirs)
+        g(view, proj, gb, pyr, out=out)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            g(view, proj, gb, pyr, out=out)
+        e1.record()
+        e1.synchronize()
+        times[t] = e0.elapsed_time(e1) / reps
+    return min(times, key=times.get), times
+
+
 def bilateral(inp, gb, out=None, radius=4, ds=10.5, ns=20.0, stream=None):
     if out is None:
         out = DImage(inp.w, inp.h, lgx.RGBA32F)

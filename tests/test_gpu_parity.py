@@ -192,7 +192,7 @@ def test_gather_slices_layout(dev, c1):
     assert back.numpy().tobytes() == a.numpy().tobytes()
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", list(range(8)))
 def test_gather_tiles_identical(dev, c1, tile):
     sc, view, proj, gb = c1[1]
     g = dev.GBufferDev.from_host(gb)

@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per unit (storage precision, no cache credit)
 BD, BR, BC, BO = 4, 8, 8, 16
 DEN_DIRS, TARGET_DIRS = 4, 256
-PLAIN_SWEEP = (32, 64, 128, 256)
+PLAIN_SWEEP = (32, 64, 128, 192, 256)
 QUALITY_SEEDS = (200, 201, 202, 203, 204, 205, 206, 207)
 BATCH_SEEDS = range(3000, 3032)
 BATCH_SIZES = ((512, 512), (1280, 720), (1920, 1080), (2560, 1440))

@@ -278,7 +278,7 @@ extern "C" int lgx_batch_autotune(lgx_batch *b, lgx_stream_t stream, char *buf, 
     std::string js = "[";
     double best = 1e30;
     int best_tile = b->c.tile, best_wg = b->c.workgroups;
-    const int wg_mult[] = {0, 1, 2, 4, 8, 16};
+    const int wg_mult[] = {0, 4, 8, 16};
     try {
       for (int t = 0; t < lgx_gather_tiles(); t++) {
         Check(lgx_gather_batch_set_tile(b->gb, t), "lgx_gather_batch_set_tile");

@@ -229,10 +229,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
   // serialising the prefetch with the matrix work.
   bf16x8 pin[IN_PER], pw[W_PER];
   uint32_t valid = 0;  // bit i: piece i of pin lies in the image
-  // one chunk and one channel split: every tile uses the same weights, which
-  // then stay in LDS from the first tile on
-  const bool keep_w = chunks == 1 && splits == 1;
-  auto fetch = [&](const Tile &tl, int cc, bool with_w) {
+  auto fetch = [&](const Tile &tl, int cc) {
     const bool from_a = cc * 32 < a.ca;
     const bool up = from_a && a.a_up;
     const int cs = from_a ? a.ca : a.cb;
@@ -250,7 +247,6 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
       pin[i] = *(const bf16x8 *)(img + off);
       valid |= (uint32_t)ok << i;
     }
-    if (!with_w) return;
     const bf16 *wsrc = a.wt + ((size_t)cc * a.cout + tl.co0) * 32;
 #pragma unroll
     for (int i = 0; i < W_PER; i++) {
@@ -262,8 +258,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
   int t = blockIdx.x;
   if (t >= T) return;  // the whole workgroup, before any barrier
   Tile cur = tile_of(t);
-  fetch(cur, 0, true);
-  bool stash_w = true;
+  fetch(cur, 0);
   for (;;) {
     f32x4 acc[WR][WP][MB];
 #pragma unroll
@@ -279,20 +274,17 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
         const bf16x8 z = {};
         if (q < IN_PIECES) *(bf16x8 *)(s_in + (q & 3) * PLANE_IN + (q >> 2) * 8) = ((valid >> i) & 1u) ? pin[i] : z;
       }
-      if (stash_w) {
 #pragma unroll
-        for (int i = 0; i < W_PER; i++) {
-          const int q = tid + i * 256;
-          if (q < W_PIECES) *(bf16x8 *)(s_w + (q & 3) * PLANE_W + (q >> 2) * 8) = pw[i];
-        }
+      for (int i = 0; i < W_PER; i++) {
+        const int q = tid + i * 256;
+        if (q < W_PIECES) *(bf16x8 *)(s_w + (q & 3) * PLANE_W + (q >> 2) * 8) = pw[i];
       }
-      stash_w = !keep_w;
       __syncthreads();
       // in flight during the matrix work below: the next chunk, or the next tile's first
       if (cc + 1 < chunks)
-        fetch(cur, cc + 1, true);
+        fetch(cur, cc + 1);
       else if (t + (int)gridDim.x < T)
-        fetch(tile_of(t + (int)gridDim.x), 0, !keep_w);
+        fetch(tile_of(t + (int)gridDim.x), 0);
 #pragma unroll
       for (int dx = 0; dx < 3; dx++) {
         bf16x8 bx[WR + 2][WP];

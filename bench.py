@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-denoised", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-train", action="store_true")
+    ap.add_argument("--gather-tile", type=int, default=-1,
+                    help="fixed gather launch shape for the plain path (default: the autotuner's choice)")
     ap.add_argument("--no-quality", action="store_true",
                     help="time the denoised path only (no error sweep against the FP64 target)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -91,7 +93,10 @@ def main():
     # timed per pass with hipEvents on the frame's stream; the gather's launch
     # shape comes from the autotuner (all shapes give identical results).
     pyr = dev.PyramidDev(g, LEVELS, sampler).build()
-    plain_tile, plain_tiles = dev.autotune_gather(g, pyr, view, proj, DIRS, STEPS, sampler)
+    if args.gather_tile >= 0:
+        plain_tile, plain_tiles = args.gather_tile, {}
+    else:
+        plain_tile, plain_tiles = dev.autotune_gather(g, pyr, view, proj, DIRS, STEPS, sampler)
     renderer = dev.Renderer(W, H, DIRS, STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_BILATERAL,
                             denoise_passes=BILAT_PASSES, denoise_radius=BILAT_RADIUS, kind="mixed", profile=True,
                             tile=plain_tile)

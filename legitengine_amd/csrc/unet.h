@@ -25,6 +25,12 @@ struct ConvArgs {
   bf16 *out1;     // channels [split, cout)
   const bf16 *mask0, *mask1;
   int tile;       // register tile: an index below conv3x3_tiles(cout), 0 = default
+  // Inference head fused into the epilogue (cout = 32, an LDS tile): instead
+  // of storing the activation, out = noisy + head_b + head_w * act per pixel.
+  const float *head_w, *head_b;  // [3][32], [3]
+  const float4 *head_noisy;      // residual, row pitch head_noisy_pitch
+  float4 *head_out;              // row pitch head_out_pitch; null = no head
+  int head_noisy_pitch, head_out_pitch;
 };
 
 struct WgradArgs {
@@ -41,6 +47,7 @@ struct WgradArgs {
 
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
 int conv3x3_tiles(int cout);  // register tiles offered for this output channel count
+int conv3x3_tile_lds(int cout, int tile);  // 1 if that tile is an LDS tile covering all cout channels
 hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s);
 hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s);
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);

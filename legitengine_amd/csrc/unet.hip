@@ -257,8 +257,8 @@ class Net {
     }
   }
 
-  // the convolutions and pools from x32 to d1
-  void body(int n, int h, int w, hipStream_t s) {
+  // the convolutions and pools from x32 to d1 (layers below `upto`)
+  void body(int n, int h, int w, hipStream_t s, int upto = 10) {
     const int h2 = h / 2, w2 = w / 2;
     fwd_layer(0, n, h, w, s);
     fwd_layer(1, n, h, w, s);
@@ -266,7 +266,7 @@ class Net {
     fwd_layer(2, n, h, w, s);
     fwd_layer(3, n, h, w, s);
     check(avgpool(e2, n, h2, w2, 64, p2, s), "avgpool");
-    for (int l = 4; l < 10; l++) fwd_layer(l, n, h, w, s);
+    for (int l = 4; l < upto; l++) fwd_layer(l, n, h, w, s);
   }
 
   void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
@@ -281,10 +281,36 @@ class Net {
   // inference straight from a G-buffer and the noisy image (same values as
   // packing the records and calling forward)
   void denoise(const lgx_image &noisy, const lgx_gbuffer &g, const lgx_image &out_, hipStream_t s) {
+    const int h = noisy.h, w = noisy.w;
     check(pack_gbuffer32(noisy, g, x32, s), "pack_gbuffer32");
-    body(1, noisy.h, noisy.w, s);
-    check(head_fwd_img(d1, noisy, params + head_w_off, params + head_b_off, out_.pitch, (float4 *)out_.ptr, s),
-          "head_fwd_img");
+    body(1, h, w, s, 9);
+    // the last convolution with the 1x1 head fused into its epilogue (an LDS
+    // tile over all 32 channels: the autotuned one, else the first offered)
+    ConvArgs c{};
+    c.a = d1a;
+    c.ca = 32;
+    c.n = 1;
+    c.h = h;
+    c.w = w;
+    c.wt = wf[9];
+    c.bias = params + b_off[9];
+    c.cout = 32;
+    c.relu = 1;
+    c.split = 32;
+    c.tile = fwd_tile[9];
+    if (!conv3x3_tile_lds(32, c.tile))
+      for (int t = 0; t < conv3x3_tiles(32); t++)
+        if (conv3x3_tile_lds(32, t)) {
+          c.tile = t;
+          break;
+        }
+    c.head_w = params + head_w_off;
+    c.head_b = params + head_b_off;
+    c.head_noisy = (const float4 *)noisy.ptr;
+    c.head_noisy_pitch = noisy.pitch;
+    c.head_out = (float4 *)out_.ptr;
+    c.head_out_pitch = out_.pitch;
+    check(conv3x3(c, s), "conv3x3 with head");
   }
 
   // Times every register tile of every forward (and, with backward, every

@@ -312,6 +312,42 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
       }
       __syncthreads();  // the next stash overwrites the tiles
     }
+    if (a.head_out) {
+      // fused inference head: each of the 4 lanes of a pixel (kg) holds 8 of
+      // its 32 channels; partial dot products are reduced across them
+#pragma unroll
+      for (int r = 0; r < WR; r++) {
+        const int y = cur.y0 + wave * WR + r;
+        if (y >= a.h) break;  // uniform across the wave
+#pragma unroll
+        for (int pb = 0; pb < WP; pb++) {
+          const int x = cur.x0 + pb * 16 + col;
+          float part[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) {
+            const int co = cur.co0 + mb * 16 + kg * 4;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              float v = acc[r][pb][mb][i] + a.bias[co + i];
+              v = v > 0.f ? v : 0.f;
+              v = bf2f(f2bf(v));  // the activation's storage precision
+#pragma unroll
+              for (int k = 0; k < 3; k++) part[k] += a.head_w[k * 32 + co + i] * v;
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            part[k] += __shfl_xor(part[k], 16);
+            part[k] += __shfl_xor(part[k], 32);
+          }
+          if (kg == 0 && x < a.w) {
+            const float4 c = a.head_noisy[(size_t)y * a.head_noisy_pitch + x];
+            a.head_out[(size_t)y * a.head_out_pitch + x] =
+                make_float4(c.x + a.head_b[0] + part[0], c.y + a.head_b[1] + part[1], c.z + a.head_b[2] + part[2], 1.f);
+          }
+        }
+      }
+    } else
 #pragma unroll
     for (int r = 0; r < WR; r++) {
       const int y = cur.y0 + wave * WR + r;
@@ -774,6 +810,13 @@ static const ConvTile *conv_tiles(int mb, int *count) {
 #undef LGX_TILES
 }
 
+int conv3x3_tile_lds(int cout, int tile) {
+  int n = 0;
+  if (cout % 16) return 0;
+  const ConvTile *t = conv_tiles(cout / 16, &n);
+  return t && tile >= 0 && tile < n && t[tile].lds && t[tile].mbw * 16 == cout;
+}
+
 int conv3x3_tiles(int cout) {
   int n = 0;
   if (cout % 16) return 0;
@@ -839,6 +882,7 @@ template <int MB> static hipError_t conv_mb(const ConvArgs &a, const ConvTile &t
 
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
   if (a.cout % 16) return hipErrorInvalidValue;
+  if (a.head_out && (a.cout != 32 || a.n != 1 || !conv3x3_tile_lds(a.cout, a.tile))) return hipErrorInvalidValue;
   const int mb = a.cout / 16;
   int n = 0;
   const ConvTile *tiles = conv_tiles(mb, &n);

@@ -191,7 +191,9 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
 // tiles blockIdx.x, blockIdx.x + gridDim.x, ...  Every output is
 // accumulated in the same order as conv3x3_kernel (chunk, dx, dy), so the
 // two kernels give bit-identical results.
-template <int MB, int WR, int WP>
+// HEAD: the inference head fused into the epilogue (its own instantiation,
+// so the plain epilogue's register budget is unaffected).
+template <int MB, int WR, int WP, bool HEAD>
 __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x, int tiles_y, int splits) {
   constexpr int TH = 4 * WR, TW = 16 * WP, IW = TW + 2, IH = TH + 2;
   constexpr int IN_PIECES = IH * IW * 4;     // 16-byte pieces of the input tile
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
       }
       __syncthreads();  // the next stash overwrites the tiles
     }
-    if (a.head_out) {
+    if constexpr (HEAD) {
       // fused inference head: each of the 4 lanes of a pixel (kg) holds 8 of
       // its 32 channels; partial dot products are reduced across them
 #pragma unroll
@@ -846,7 +848,15 @@ template <int MBW, int WR, int WP> static void launch_conv_lds(const ConvArgs &a
   const int tx = (a.w + 16 * WP - 1) / (16 * WP), ty = (a.h + 4 * WR - 1) / (4 * WR), splits = a.cout / (16 * MBW);
   const long tiles = (long)tx * ty * a.n * splits;
   long grid = per_cu > 0 ? std::min(tiles, (long)per_cu * cu_count()) : tiles;
-  hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty, splits);
+  if constexpr (MBW == 2) {
+    if (a.head_out) {
+      hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, true>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty,
+                         splits);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, false>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty,
+                     splits);
 }
 
 template <int MB> static hipError_t conv_mb(const ConvArgs &a, const ConvTile &t, hipStream_t s) {

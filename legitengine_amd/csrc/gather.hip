@@ -240,7 +240,8 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
         const float *dl = a.pd[l];
         const void *rl = a.pr[l];
         const int dpitch = a.pd_pitch[l], rpitch = a.pr_pitch[l];
-        const T flw = T(a.lw[l]), flh = T(a.lh[l]), flw1 = T(a.lw[l] - 1), flh1 = T(a.lh[l] - 1);
+        const T flw = T(a.lw[l]), flh = T(a.lh[l]);
+        const int lw1 = a.lw[l] - 1, lh1 = a.lh[l] - 1;
         for (; j < jend; j++) {
           // every active lane is at the same step (all start a level at its
           // first step and advance together), so the table read is uniform
@@ -249,8 +250,10 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
           const T qx = cx + dx * off, qy = cy + dy * off;  // :218
           const T sqx = div_extent(qx, a.W, a.rW), sqy = div_extent(qy, a.H, a.rH);  // :219
           if (!(sqx > T(0) && sqx < T(1) && sqy > T(0) && sqy < T(1))) goto dir_done;
-          const int ix = (int)min(max(floor(sqx * flw), T(0)), flw1);
-          const int iy = (int)min(max(floor(sqy * flh), T(0)), flh1);
+          // on screen: sq > 0, so floor(sq * extent) >= 0; the product can round
+          // up to the extent itself, hence the upper clamp (in integers)
+          const int ix = min((int)floor(sqx * flw), lw1);
+          const int iy = min((int)floor(sqy * flh), lh1);
           const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
           const T r = T(dl[ti]);
           // :241-252 sample delta projected on (eye, tangent)

@@ -191,10 +191,10 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
 // tiles blockIdx.x, blockIdx.x + gridDim.x, ...  Every output is
 // accumulated in the same order as conv3x3_kernel (chunk, dx, dy), so the
 // two kernels give bit-identical results.
-// HEAD: the inference head fused into the epilogue (its own instantiation,
-// so the plain epilogue's register budget is unaffected).
+// HEAD: the inference head fused into the epilogue (its own instantiation;
+// needs MB = 2, i.e. all 32 channels of a pixel in one workgroup).
 template <int MB, int WR, int WP, bool HEAD>
-__global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x, int tiles_y, int splits) {
+__global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x, int tiles_y, int splits) {
   constexpr int TH = 4 * WR, TW = 16 * WP, IW = TW + 2, IH = TH + 2;
   constexpr int IN_PIECES = IH * IW * 4;     // 16-byte pieces of the input tile
   constexpr int W_PIECES = 9 * MB * 16 * 4;  // of the chunk's weights
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
       // in flight during the matrix work below: the next chunk, or the next tile's first
       if (cc + 1 < chunks)
         fetch(cur, cc + 1);
-      else if (t + (int)gridDim.x < T)
+      else if (!HEAD && t + (int)gridDim.x < T)  // a head launch has one tile per workgroup
         fetch(tile_of(t + (int)gridDim.x), 0);
 #pragma unroll
       for (int dx = 0; dx < 3; dx++) {
@@ -315,40 +315,48 @@ __global__ __launch_bounds__(256, 2) void conv3x3_lds_kernel(const ConvArgs a, i
       __syncthreads();  // the next stash overwrites the tiles
     }
     if constexpr (HEAD) {
-      // fused inference head: each of the 4 lanes of a pixel (kg) holds 8 of
-      // its 32 channels; partial dot products are reduced across them
+      // fused inference head: the tile's activations (BF16, as the unfused
+      // path stores them) go to LDS, then one thread per pixel forms
+      // noisy + b + W * act in the unfused head's order (bit-identical)
+      bf16 *act = s_in;  // [TH][TW][32]; the input tile is no longer read
+      float *hw = (float *)s_w;  // head weights [3][32] and bias [3]; the chunk weights are no longer read
+      if (tid < 99) hw[tid] = tid < 96 ? a.head_w[tid] : a.head_b[tid - 96];
 #pragma unroll
-      for (int r = 0; r < WR; r++) {
-        const int y = cur.y0 + wave * WR + r;
-        if (y >= a.h) break;  // uniform across the wave
+      for (int r = 0; r < WR; r++)
 #pragma unroll
-        for (int pb = 0; pb < WP; pb++) {
-          const int x = cur.x0 + pb * 16 + col;
-          float part[3] = {0.f, 0.f, 0.f};
+        for (int pb = 0; pb < WP; pb++)
 #pragma unroll
           for (int mb = 0; mb < MB; mb++) {
-            const int co = cur.co0 + mb * 16 + kg * 4;
+            const int co = mb * 16 + kg * 4;
+            typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            bf16x4 o;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
               float v = acc[r][pb][mb][i] + a.bias[co + i];
-              v = v > 0.f ? v : 0.f;
-              v = bf2f(f2bf(v));  // the activation's storage precision
+              o[i] = f2bf(v > 0.f ? v : 0.f);
+            }
+            *(bf16x4 *)(act + ((wave * WR + r) * TW + pb * 16 + col) * 32 + co) = o;
+          }
+      __syncthreads();
+      for (int p = tid; p < TH * TW; p += 256) {
+        const int y = cur.y0 + p / TW, x = cur.x0 + p % TW;
+        if (y < a.h && x < a.w) {
+          const float4 c = a.head_noisy[(size_t)y * a.head_noisy_pitch + x];
+          float o[3] = {c.x + hw[96], c.y + hw[97], c.z + hw[98]};
+#pragma unroll 1
+          for (int q = 0; q < 4; q++) {
+            const bf16x8 v = *(const bf16x8 *)(act + p * 32 + q * 8);
 #pragma unroll
-              for (int k = 0; k < 3; k++) part[k] += a.head_w[k * 32 + co + i] * v;
+            for (int i = 0; i < 8; i++) {
+              const float f = bf2f(v[i]);
+#pragma unroll
+              for (int k = 0; k < 3; k++) o[k] += hw[k * 32 + q * 8 + i] * f;
             }
           }
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            part[k] += __shfl_xor(part[k], 16);
-            part[k] += __shfl_xor(part[k], 32);
-          }
-          if (kg == 0 && x < a.w) {
-            const float4 c = a.head_noisy[(size_t)y * a.head_noisy_pitch + x];
-            a.head_out[(size_t)y * a.head_out_pitch + x] =
-                make_float4(c.x + a.head_b[0] + part[0], c.y + a.head_b[1] + part[1], c.z + a.head_b[2] + part[2], 1.f);
-          }
+          a.head_out[(size_t)y * a.head_out_pitch + x] = make_float4(o[0], o[1], o[2], 1.f);
         }
       }
+      __syncthreads();  // the next tile's stash overwrites the activations
     } else
 #pragma unroll
     for (int r = 0; r < WR; r++) {
@@ -849,8 +857,8 @@ template <int MBW, int WR, int WP> static void launch_conv_lds(const ConvArgs &a
   const long tiles = (long)tx * ty * a.n * splits;
   long grid = per_cu > 0 ? std::min(tiles, (long)per_cu * cu_count()) : tiles;
   if constexpr (MBW == 2) {
-    if (a.head_out) {
-      hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, true>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty,
+    if (a.head_out) {  // one tile per workgroup: nothing is prefetched across the head's epilogue
+      hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, true>), dim3((unsigned)tiles), dim3(256), 0, s, a, tx, ty,
                          splits);
       return;
     }

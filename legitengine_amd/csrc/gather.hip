@@ -29,6 +29,9 @@
 
 namespace lgx {
 
+// Buffer resource word 3 of the gfx9 family (raw, untyped 32-bit access).
+static constexpr int kBufferConfig = 0x00020000;
+
 template <class T> struct GatherArgs {
   T inv_vp[16];
   T cam[3];
@@ -240,6 +243,10 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
         const float *dl = a.pd[l];
         const void *rl = a.pr[l];
         const int dpitch = a.pd_pitch[l], rpitch = a.pr_pitch[l];
+        // the level's depth as a buffer resource: the per-step fetch takes a
+        // 32-bit byte offset against a scalar base (no 64-bit address math)
+        const __amdgpu_buffer_rsrc_t drs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)dl, (short)0, a.lh[l] * dpitch * 4, kBufferConfig);
         const T flw = T(a.lw[l]), flh = T(a.lh[l]);
         const int lw1 = a.lw[l] - 1, lh1 = a.lh[l] - 1;
         for (; j < jend; j++) {
@@ -255,7 +262,7 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
           const int ix = min((int)floor(sqx * flw), lw1);
           const int iy = min((int)floor(sqy * flh), lh1);
           const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
-          const T r = T(dl[ti]);
+          const T r = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0)));
           // :241-252 sample delta projected on (eye, tangent)
           const T len2 = fma_(off, fma_(off, vv, uv2), uu);
           const T ri = r * fast_rsqrt(len2);

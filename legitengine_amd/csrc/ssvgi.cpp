@@ -190,7 +190,7 @@ void SsvgiRenderer::RenderFrame(const SsvgiFrame &f) {
   rg_->UpdateExternalImageView(v.normal.Id(), g.normal);
   rg_->UpdateExternalImageView(v.albedo.Id(), g.albedo);
 
-  // SSVGIRenderer.h:207-208 mips; Hi-Z max for depth in north-star mode
+  // SSVGIRenderer.h:207-208 mips; Hi-Z max for depth in the Hi-Z point-fetch mode
   mipBuilder_.BuildMips(rg_, v.depth,
                         c.sampler == LGX_SAMPLER_HIZ_POINT ? MipBuilder::FilterTypes::Depth : MipBuilder::FilterTypes::Avg);
   mipBuilder_.BuildMips(rg_, v.radiance, MipBuilder::FilterTypes::Avg);

@@ -222,7 +222,7 @@ template <class T> void sample_trilinear(const Level *lv, int nl, int c, T sx, T
 }
 
 /* Hi-Z point fetch: level clamp(floor(lod), 0, levels-1), nearest texel
- * floor(s * W_l) clamped to the level (north-star design mode). */
+ * floor(s * W_l) clamped to the level (the Hi-Z point-fetch mode). */
 template <class T> void sample_point(const Level *lv, int nl, int c, T sx, T sy, T lod, T *out) {
   int l = 0;
   if (lod > T(0)) l = (int)std::min<T>(std::floor(lod), T(nl - 1));
@@ -280,7 +280,7 @@ int nthreads(int t) {
 }
 
 /* One pixel of indirectLighting.frag main (:114-272), in reference mode
- * (sampler 0, D from params, no cap) or north-star mode (Hi-Z point fetch,
+ * (sampler 0, D from params, no cap) or the Hi-Z mode (point fetch,
  * step cap S).  The centre is reconstructed from the radial distance the
  * G-buffer stores (gBufferBuilder.frag:31,37): P = cam + ray(s) * r, the
  * same reconstruction the shader applies to every sample (:241).  The

@@ -23,7 +23,7 @@ extern "C" {
 #endif
 
 /* Gather parameters (indirectLighting.frag:4-9 uniform block + the constants
- * the shader hard-codes, promoted to parameters as the north-star mode needs). */
+ * the shader hard-codes, promoted to parameters as the Hi-Z point-fetch mode needs). */
 typedef struct {
   float view[16];   /* world -> camera, column-major (glm layout) */
   float proj[16];   /* camera -> ndc, column-major */

@@ -495,14 +495,28 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
 
 // Per-channel sums of an NHWC tensor (bias gradients), atomically added.
 __global__ __launch_bounds__(256) void channel_sum_kernel(const bf16 *x, int c, size_t npix, float *out) {
-  // thread t sums channel t % c over pixels (t / c), (t / c) + stride, ...
-  const int ch = threadIdx.x % c, lanes_per = 256 / c;
-  const size_t p0 = (size_t)blockIdx.x * lanes_per + threadIdx.x / c;
-  const size_t step = (size_t)gridDim.x * lanes_per;
-  if ((int)threadIdx.x >= lanes_per * c) return;
-  float s = 0.f;
-  for (size_t p = p0; p < npix; p += step) s += bf2f(x[p * c + ch]);
-  atomicAdd(&out[ch], s);
+  // thread: one 8-channel piece (gi) of pixels pi, pi + per, ... (16-byte
+  // loads); the block then adds the threads of each piece in LDS and issues
+  // one atomic per channel
+  __shared__ float red[256][9];
+  const int groups = c >> 3, per = 256 / groups;
+  const int gi = threadIdx.x % groups, pi = threadIdx.x / groups;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (pi < per)
+    for (size_t p = (size_t)blockIdx.x * per + pi; p < npix; p += (size_t)gridDim.x * per) {
+      const bf16x8 v = *(const bf16x8 *)(x + p * c + gi * 8);
+#pragma unroll
+      for (int i = 0; i < 8; i++) acc[i] += bf2f(v[i]);
+    }
+#pragma unroll
+  for (int i = 0; i < 8; i++) red[threadIdx.x][i] = acc[i];
+  __syncthreads();
+  if ((int)threadIdx.x < c) {
+    const int g = threadIdx.x >> 3, i = threadIdx.x & 7;
+    float sum = 0.f;
+    for (int q = 0; q < per; q++) sum += red[q * groups + g][i];
+    atomicAdd(&out[threadIdx.x], sum);
+  }
 }
 
 // ------------------------------------------------------------ glue kernels
@@ -679,37 +693,48 @@ __global__ __launch_bounds__(256) void head_fwd_img_kernel(const bf16 *d1, const
 __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const float4 *out, const float4 *target,
                                                        const float *wh, size_t npix, float inv_count, bf16 *dd1,
                                                        float *dwh, float *dbh, float *loss) {
+  // each thread walks pixels p, p + stride, ... keeping its head-gradient
+  // partial sums in registers, then adds them once to the block's LDS sums
   __shared__ float sw[96 + 3 + 1];
   for (int i = threadIdx.x; i < 100; i += 256) sw[i] = 0.f;
   __syncthreads();
-  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-  float g[3] = {0.f, 0.f, 0.f};
-  float l = 0.f;
-  if (p < npix) {
+  float aw[96], ab[3] = {0.f, 0.f, 0.f}, al = 0.f;
+#pragma unroll
+  for (int i = 0; i < 96; i++) aw[i] = 0.f;
+  for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < npix; p += (size_t)gridDim.x * 256) {
     const float4 o = out[p], t = target[p];
     const float df[3] = {o.x - t.x, o.y - t.y, o.z - t.z};
+    float g[3];
+#pragma unroll
     for (int c = 0; c < 3; c++) {
       g[c] = (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f)) * inv_count;
-      l += fabsf(df[c]);
+      al += fabsf(df[c]);
+      ab[c] += g[c];
     }
+#pragma unroll
     for (int q = 0; q < 4; q++) {
       const bf16x8 v = *(const bf16x8 *)(d1 + p * 32 + q * 8);
       bf16x8 o8;
+#pragma unroll
       for (int i = 0; i < 8; i++) {
         const int k = q * 8 + i;
         const float f = bf2f(v[i]);
-        float s = 0.f;
+        float sacc = 0.f;
+#pragma unroll
         for (int c = 0; c < 3; c++) {
-          s += wh[c * 32 + k] * g[c];
-          atomicAdd(&sw[c * 32 + k], g[c] * f);
+          sacc += wh[c * 32 + k] * g[c];
+          aw[c * 32 + k] += g[c] * f;
         }
-        o8[i] = f > 0.f ? f2bf(s) : f2bf(0.f);
+        o8[i] = f > 0.f ? f2bf(sacc) : f2bf(0.f);
       }
       *(bf16x8 *)(dd1 + p * 32 + q * 8) = o8;
     }
-    for (int c = 0; c < 3; c++) atomicAdd(&sw[96 + c], g[c]);
-    atomicAdd(&sw[99], l);
   }
+#pragma unroll
+  for (int i = 0; i < 96; i++) atomicAdd(&sw[i], aw[i]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) atomicAdd(&sw[96 + c], ab[c]);
+  atomicAdd(&sw[99], al);
   __syncthreads();
   for (int i = threadIdx.x; i < 96; i += 256) atomicAdd(&dwh[i], sw[i]);
   if (threadIdx.x < 3) atomicAdd(&dbh[threadIdx.x], sw[96 + threadIdx.x]);
@@ -930,7 +955,10 @@ hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s) {
 }
 
 hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s) {
-  hipLaunchKernelGGL(channel_sum_kernel, dim3(grid_for(npix, 2048)), dim3(256), 0, s, x, c, npix, out);
+  if (c % 8 || c > 256 || 256 % (c / 8)) return hipErrorInvalidValue;
+  const size_t per = 256 / (c / 8);
+  hipLaunchKernelGGL(channel_sum_kernel, dim3((unsigned)std::min<size_t>(1024, (npix + per - 1) / per)), dim3(256), 0,
+                     s, x, c, npix, out);
   return hipGetLastError();
 }
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s) {
@@ -998,8 +1026,10 @@ hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const flo
 }
 hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
                     float inv_count, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s) {
-  hipLaunchKernelGGL(head_bwd_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, d1, out, target, wh, npix,
-                     inv_count, dd1, dwh, dbh, loss);
+  // about 16 pixels per thread: the per-thread partial sums amortise the block's LDS atomics
+  const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (npix + 4095) / 4096));
+  hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks), dim3(256), 0, s, d1, out, target, wh, npix, inv_count, dd1, dwh,
+                     dbh, loss);
   return hipGetLastError();
 }
 hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,

@@ -264,11 +264,19 @@ int64_t lgx_unet_param_count(void);
 int lgx_unet_create(int32_t max_n, int32_t max_h, int32_t max_w, uint64_t seed, lgx_unet **out);
 /* out: device RGBA32F [n][h][w] (alpha 1). */
 int lgx_unet_forward(lgx_unet *u, const float *x12, int32_t n, int32_t h, int32_t w, float *out, lgx_stream_t stream);
-/* Forward + L1-loss backward against target (device RGBA32F [n][h][w]);
- * gradients kept inside; loss_out (device float, optional) = mean |out - target| over rgb. */
+/* Forward + loss backward against target (device RGBA32F [n][h][w]);
+ * gradients kept inside; loss_out (device float, optional) = the mean over
+ * rgb of |out - target| (LGX_UNET_LOSS_L1, the default) or of
+ * (out - target)^2 (LGX_UNET_LOSS_L2, see lgx_unet_set_loss). */
 int lgx_unet_backward(lgx_unet *u, const float *x12, const float *target, int32_t n, int32_t h, int32_t w,
                       float *loss_out, lgx_stream_t stream);
 int lgx_unet_adam_step(lgx_unet *u, float lr, lgx_stream_t stream);
+/* Training loss of later backward calls: mean absolute error (the default)
+ * or mean squared error, whose minimiser is the conditional mean that a
+ * relative-L2 metric scores. */
+#define LGX_UNET_LOSS_L1 0
+#define LGX_UNET_LOSS_L2 1
+int lgx_unet_set_loss(lgx_unet *u, int32_t loss);
 int lgx_unet_train_step(lgx_unet *u, const float *x12, const float *target, int32_t n, int32_t h, int32_t w, float lr,
                         float *loss_out, lgx_stream_t stream);
 /* Device pointer of the last forward's output inside the network (RGBA32F). */

@@ -41,14 +41,17 @@ struct WgradArgs {
   const bf16 *dy;  // NHWC cout
   int cout;
   int n, h, w;
-  int rows;        // rows per workgroup
-  float *dw;       // [9][ca+cb][cout], accumulated
+  int rows;        // rows per workgroup (set by conv3x3_wgrad)
+  float *dw;       // [9][ca+cb][cout], overwritten
+  float *part;     // scratch for the per-workgroup partial sums
+  size_t part_bytes;
 };
 
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
 int conv3x3_tiles(int cout);  // register tiles offered for this output channel count
 int conv3x3_tile_lds(int cout, int tile);  // 1 if that tile is an LDS tile covering all cout channels
-hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s);
+hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s);
+size_t conv3x3_wgrad_scratch_bytes();
 hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s);
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);
 hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12, hipStream_t s);
@@ -64,7 +67,7 @@ hipError_t upsample_bwd(const bf16 *dfull, const bf16 *act, int n, int hh, int w
 hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, int w,
                     int out_pitch, float4 *out, hipStream_t s);
 hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
-                    float inv_count, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s);
+                    float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s);
 hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
                 float bc1, float bc2, hipStream_t s);
 hipError_t fill(float *p, size_t n, float v, hipStream_t s);

@@ -65,6 +65,9 @@ class Net {
   // activations
   bf16 *x32, *e1a, *e1, *p1, *e2a, *e2, *p2, *b1, *b2, *d2a, *d2, *d1a, *d1;
   float *out, *x12_scratch, *loss;
+  float *wpart;  // weight-gradient partial sums
+  size_t wpart_bytes;
+  int loss_kind = LGX_UNET_LOSS_L1;
   // gradients of activations
   bf16 *g_d1, *g_d1a, *g_upd2, *g_e1c, *g_d2, *g_d2a, *g_upb2, *g_e2c, *g_b2, *g_b1, *g_p2, *g_e2, *g_e2a, *g_p1, *g_e1,
       *g_e1a;
@@ -130,6 +133,8 @@ class Net {
     out = (float *)alloc(P * 16);
     x12_scratch = (float *)alloc((size_t)h * w * 48);
     loss = (float *)alloc(4);
+    wpart_bytes = conv3x3_wgrad_scratch_bytes();
+    wpart = (float *)alloc(wpart_bytes);
     g_d1 = A(P, 32);
     g_d1a = A(P, 32);
     g_upd2 = A(P, 64);
@@ -215,11 +220,9 @@ class Net {
     g.n = n;
     g.h = h;
     g.w = w;
-    const int groups = (ca + cb) / 32;
-    long blocks_per_row = (long)((w + 63) / 64) * n * groups;
-    int rows = (int)((long)h * blocks_per_row / 1024);
-    g.rows = rows < 1 ? 1 : (rows > 16 ? 16 : rows);
     g.dw = dw[l];
+    g.part = wpart;
+    g.part_bytes = wpart_bytes;
     check(conv3x3_wgrad(g, s), "conv3x3_wgrad");
     check(channel_sum(dy, kLayers[l].cout, (size_t)n * h * w, grads + b_off[l], s), "channel_sum");
   }
@@ -372,8 +375,7 @@ class Net {
     forward(x12, n, h, w, out, s);
     check(fill(grads, np, 0.f, s), "fill");
     check(fill(loss, 1, 0.f, s), "fill");
-    for (int l = 0; l < 10; l++) check(fill(dw[l], (size_t)9 * kLayers[l].cin_pad * kLayers[l].cout, 0.f, s), "fill");
-    check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P), g_d1,
+    check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P), loss_kind == LGX_UNET_LOSS_L2, g_d1,
                    grads + head_w_off, grads + head_b_off, loss, s),
           "head_bwd");
     // d1 = relu(conv9(d1a))
@@ -507,6 +509,12 @@ extern "C" int lgx_unet_backward(lgx_unet *u, const float *x12, const float *tar
     if (loss_out)
       N.check(hipMemcpyAsync(loss_out, N.loss, 4, hipMemcpyDeviceToDevice, (hipStream_t)stream), "hipMemcpyAsync");
   });
+}
+
+extern "C" int lgx_unet_set_loss(lgx_unet *u, int32_t loss) {
+  if (!u || !u->net || (loss != LGX_UNET_LOSS_L1 && loss != LGX_UNET_LOSS_L2)) return LGX_EINVAL;
+  u->net->loss_kind = loss;
+  return LGX_OK;
 }
 
 extern "C" int lgx_unet_adam_step(lgx_unet *u, float lr, lgx_stream_t stream) {

@@ -411,15 +411,21 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
 // Weight and bias gradient of a 3x3 convolution:
 //   dW[tap][ci][co] += sum_p X[p + shift(tap)][ci] * dY[p][co],  db[co] += sum_p dY[p][co]
 // One workgroup: one image, one 32-channel group of the input, a 64-pixel
-// column band and `rows` rows.  Per row, the three shifted input rows and the
-// output-gradient row are staged in LDS transposed (channel-major) so that
-// MFMA operands are 8 consecutive pixels of one channel; K runs over pixels.
-// Tiles (tap, co block, ci block) are spread over the 4 waves; partial sums
-// are added to the FP32 gradient with atomics at the end.
+// column band and `rows` rows, taken RB rows per round.  Per round the RB + 2
+// input rows and the RB output-gradient rows are staged in LDS transposed
+// (channel-major, written as pixel pairs) so that MFMA operands are 8
+// consecutive pixels of one channel; K runs over pixels.  The horizontally
+// shifted input operands are two aligned 16-byte reads and an element shift.
+// Tiles (tap, co block, ci block) are spread over the 4 waves; each
+// workgroup stores its partial sums (no atomics) to its own slot of the
+// scratch buffer, part[group][slot][tap][ci % 32][co], and
+// wgrad_reduce_kernel adds the slots in a fixed order.
 template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const WgradArgs a) {
   constexpr int XS = 64 + 16;  // staged x range [x0 - 8, x0 + 72): keeps dx = 0 reads 16-byte aligned
-  __shared__ __align__(16) bf16 xs[3][32][XS];
-  __shared__ __align__(16) bf16 ds[MB * 16][64];
+  constexpr int RB = 2;        // output rows per staging round
+  constexpr int COUT = MB * 16;
+  __shared__ __align__(16) bf16 xs[RB + 2][32][XS];
+  __shared__ __align__(16) bf16 ds[RB][COUT][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kg = lane >> 4;
   const int groups = (a.ca + a.cb) >> 5;
@@ -432,64 +438,114 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
   f32x4 acc[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int cout = MB * 16;
-  for (int r = 0; r < a.rows; r++) {
-    const int y = y0 + r;
-    if (y >= a.h) break;  // uniform across the workgroup
-    // stage input rows y-1, y, y+1 (32 channels of group g), transposed
-    for (int e = threadIdx.x; e < 3 * XS * 4; e += 256) {
-      const int q = e & 3, px = (e >> 2) % XS, dy = (e >> 2) / XS;
-      bf16x8 v = load8(in, n, y + dy - 1, x0 - 8 + px, a.h, a.w, g * 32 + q * 8);
+  typedef unsigned short u16;
+  auto pair = [](bf16 lo, bf16 hi) {
+    return (uint32_t)__builtin_bit_cast(u16, lo) | ((uint32_t)__builtin_bit_cast(u16, hi) << 16);
+  };
+  for (int r0 = 0; r0 < a.rows; r0 += RB) {
+    const int yr = y0 + r0;
+    if (yr >= a.h) break;  // uniform across the workgroup
+    const int nr = min(RB, min(a.rows - r0, a.h - yr));
+    // input rows yr - 1 .. yr + nr (32 channels of group g), transposed, pixel pairs
+    for (int e = threadIdx.x; e < (RB + 2) * (XS / 2) * 4; e += 256) {
+      const int q = e & 3, pp = (e >> 2) % (XS / 2), rr = (e >> 2) / (XS / 2);
+      const int px = pp * 2;
+      bf16x8 v0 = {}, v1 = {};
+      if (rr < nr + 2) {
+        v0 = load8(in, n, yr + rr - 1, x0 - 8 + px, a.h, a.w, g * 32 + q * 8);
+        v1 = load8(in, n, yr + rr - 1, x0 - 8 + px + 1, a.h, a.w, g * 32 + q * 8);
+      }
 #pragma unroll
-      for (int i = 0; i < 8; i++) xs[dy][q * 8 + i][px] = v[i];
+      for (int i = 0; i < 8; i++) *(uint32_t *)&xs[rr][q * 8 + i][px] = pair(v0[i], v1[i]);
     }
-    // stage the output-gradient row, transposed
-    for (int e = threadIdx.x; e < 64 * (cout / 8); e += 256) {
-      const int q = e % (cout / 8), px = e / (cout / 8);
-      const int x = x0 + px;
-      bf16x8 v = {};
-      if (x < a.w) v = *(const bf16x8 *)(a.dy + (((size_t)n * a.h + y) * a.w + x) * cout + q * 8);
+    // output-gradient rows yr .. yr + nr - 1, transposed, pixel pairs
+    for (int e = threadIdx.x; e < RB * 32 * (COUT / 8); e += 256) {
+      const int q = e % (COUT / 8), pp = (e / (COUT / 8)) % 32, rr = e / (COUT / 8) / 32;
+      const int x = x0 + pp * 2;
+      bf16x8 v0 = {}, v1 = {};
+      if (rr < nr) {
+        const bf16 *row = a.dy + ((size_t)n * a.h + yr + rr) * a.w * COUT + q * 8;
+        if (x < a.w) v0 = *(const bf16x8 *)(row + (size_t)x * COUT);
+        if (x + 1 < a.w) v1 = *(const bf16x8 *)(row + (size_t)(x + 1) * COUT);
+      }
 #pragma unroll
-      for (int i = 0; i < 8; i++) ds[q * 8 + i][px] = v[i];
+      for (int i = 0; i < 8; i++) *(uint32_t *)&ds[rr][q * 8 + i][pp * 2] = pair(v0[i], v1[i]);
     }
     __syncthreads();
 #pragma unroll
     for (int ti = 0; ti < TPW; ti++) {
       const int t = wave + ti * 4;
       if (t < NT) {
-      const int tap = t / (MB * 2), rem = t % (MB * 2), mb = rem >> 1, cb = rem & 1;
-      const int dy = tap / 3, dx = tap % 3 - 1;
+        const int tap = t / (MB * 2), rem = t % (MB * 2), mb = rem >> 1, cb = rem & 1;
+        const int dy = tap / 3, dx = tap % 3 - 1;
+        for (int rr = 0; rr < nr; rr++) {
 #pragma unroll
-      for (int kc = 0; kc < 2; kc++) {
-        const int p0 = kc * 32 + kg * 8;
-        const bf16x8 av = *(const bf16x8 *)&ds[mb * 16 + col][p0];  // A: co x pixels
-        bf16x8 bv;                                                    // B: pixels x ci
-        const bf16 *src = &xs[dy][cb * 16 + col][p0 + 8 + dx];
-        if (dx == 0) {
-          bv = *(const bf16x8 *)src;
-        } else {
+          for (int kc = 0; kc < 2; kc++) {
+            const int p0 = kc * 32 + kg * 8;
+            const bf16x8 av = *(const bf16x8 *)&ds[rr][mb * 16 + col][p0];  // A: co x pixels
+            const bf16 *base = &xs[rr + dy][cb * 16 + col][p0 + 8];         // B: pixels x ci, 16-byte aligned
+            bf16x8 bv;
+            if (dx == 0) {
+              bv = *(const bf16x8 *)base;
+            } else if (dx < 0) {  // elements base[-1 .. 6]
+              const bf16x8 lo = *(const bf16x8 *)(base - 8), hi = *(const bf16x8 *)base;
+              bv[0] = lo[7];
 #pragma unroll
-          for (int i = 0; i < 8; i++) bv[i] = src[i];
+              for (int i = 1; i < 8; i++) bv[i] = hi[i - 1];
+            } else {  // elements base[1 .. 8]
+              const bf16x8 lo = *(const bf16x8 *)base, hi = *(const bf16x8 *)(base + 8);
+#pragma unroll
+              for (int i = 0; i < 7; i++) bv[i] = lo[i + 1];
+              bv[7] = hi[0];
+            }
+            acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[ti], 0, 0, 0);
+          }
         }
-        acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[ti], 0, 0, 0);
-      }
       }
     }
     __syncthreads();
   }
-  const int cin = a.ca + a.cb;
+  const size_t slots = (size_t)a.n * gridDim.y * gridDim.x;
+  const size_t slot = ((size_t)n * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  float *dst = a.part + ((size_t)g * slots + slot) * (9 * 32 * COUT);
 #pragma unroll
   for (int ti = 0; ti < TPW; ti++) {
     const int t = wave + ti * 4;
     if (t < NT) {
       const int tap = t / (MB * 2), rem = t % (MB * 2), mb = rem >> 1, cb = rem & 1;
-      const int ci = g * 32 + cb * 16 + col;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int co = mb * 16 + kg * 4 + i;
-        atomicAdd(&a.dw[((size_t)tap * cin + ci) * cout + co], acc[ti][i]);
-      }
+      *(f32x4 *)&dst[(tap * 32 + cb * 16 + col) * COUT + mb * 16 + kg * 4] = acc[ti];
     }
+  }
+}
+
+// dw[tap][g * 32 + cl][co] = sum over slots of part[g][slot][tap][cl][co].
+// A workgroup holds 256 / S float4 elements x S slices; slice k sums slots
+// k, k + S, ... and the S slice sums are added in a fixed tree, so the
+// gradient does not depend on the order in which workgroups ran.
+template <int S>
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const f32x4 *part, int slots, int per4, int groups,
+                                                          int cout, int cin, float *dw) {
+  constexpr int E = 256 / S;
+  __shared__ f32x4 red[S][E];
+  const int el = threadIdx.x % E, sl = threadIdx.x / E;
+  const size_t e = (size_t)blockIdx.x * E + el, total = (size_t)groups * per4;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  if (e < total) {
+    const size_t g = e / per4, f = e % per4;
+    const f32x4 *src = part + g * slots * per4 + f;
+    for (int k = sl; k < slots; k += S) acc += src[(size_t)k * per4];
+  }
+  red[sl][el] = acc;
+  __syncthreads();
+#pragma unroll
+  for (int st = S / 2; st > 0; st >>= 1) {
+    if (sl < st) red[sl][el] += red[sl + st][el];
+    __syncthreads();
+  }
+  if (sl == 0 && e < total) {
+    const int g = (int)(e / per4), f = (int)(e % per4) * 4;
+    const int tap = f / (32 * cout), rem = f % (32 * cout), cl = rem / cout, co = rem % cout;
+    *(f32x4 *)&dw[((size_t)tap * cin + g * 32 + cl) * cout + co] = red[0][el];
   }
 }
 
@@ -688,11 +744,12 @@ __global__ __launch_bounds__(256) void head_fwd_img_kernel(const bf16 *d1, const
   out[(size_t)y * out_pitch + x] = make_float4(o[0], o[1], o[2], 1.f);
 }
 
-// L1 loss head backward: g = sign(out - target) / count per channel;
-// dW_h, db_h accumulated per workgroup then atomically; dd1 = mask(d1 > 0) W_h^T g.
+// Loss head backward.  Mean L1: g = sign(out - target) / count per channel;
+// mean squared error (l2): g = 2 (out - target) / count.  dW_h, db_h
+// accumulated per workgroup then atomically; dd1 = mask(d1 > 0) W_h^T g.
 __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const float4 *out, const float4 *target,
-                                                       const float *wh, size_t npix, float inv_count, bf16 *dd1,
-                                                       float *dwh, float *dbh, float *loss) {
+                                                       const float *wh, size_t npix, float inv_count, int l2,
+                                                       bf16 *dd1, float *dwh, float *dbh, float *loss) {
   // each thread walks pixels p, p + stride, ... keeping its head-gradient
   // partial sums in registers, then adds them once to the block's LDS sums
   __shared__ float sw[96 + 3 + 1];
@@ -707,8 +764,8 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const flo
     float g[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      g[c] = (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f)) * inv_count;
-      al += fabsf(df[c]);
+      g[c] = (l2 ? 2.f * df[c] : (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f))) * inv_count;
+      al += l2 ? df[c] * df[c] : fabsf(df[c]);
       ab[c] += g[c];
     }
 #pragma unroll
@@ -942,15 +999,53 @@ hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
   }
 }
 
-hipError_t conv3x3_wgrad(const WgradArgs &a, hipStream_t s) {
-  const int groups = (a.ca + a.cb) / 32;
-  dim3 grid((a.w + 63) / 64, (a.h + a.rows - 1) / a.rows, a.n * groups), block(256);
+// Weight-gradient workgroups per launch by output-channel count: each stores
+// 9 * 32 * cout FP32 partial sums, about 38 MB per launch at these counts.
+static int wgrad_target_blocks(int cout) { return cout <= 32 ? 1024 : cout <= 64 ? 512 : 256; }
+
+size_t conv3x3_wgrad_scratch_bytes() {
+  size_t m = 0;
+  for (int cout : {32, 64, 128}) m = std::max(m, (size_t)2 * wgrad_target_blocks(cout) * 9 * 32 * cout * 4);
+  return m;
+}
+
+hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
+  if (a.cout != 32 && a.cout != 64 && a.cout != 128) return hipErrorInvalidValue;
+  if ((a.ca + a.cb) % 32 || a.ca % 32 || a.n < 1 || a.h < 1 || a.w < 1 || !a.part) return hipErrorInvalidValue;
+  const int groups = (a.ca + a.cb) / 32, bands = (a.w + 63) / 64, target = wgrad_target_blocks(a.cout);
+  const long segs = (long)bands * a.n * groups;  // workgroups per row of rows
+  // rows per workgroup: even, about `target` workgroups, at most 2 * target
+  int rows = (int)std::min<long>(a.h, std::max<long>(2, ((long)a.h * segs + target - 1) / target));
+  rows = (rows + 1) & ~1;
+  long blocks = segs * ((a.h + rows - 1) / rows);
+  while (blocks > 2L * target && rows < a.h) {
+    rows += 2;
+    blocks = segs * ((a.h + rows - 1) / rows);
+  }
+  const size_t per = (size_t)9 * 32 * a.cout;
+  if ((size_t)blocks * per * 4 > a.part_bytes) return hipErrorInvalidValue;
+  a.rows = rows;
+  dim3 grid(bands, (a.h + rows - 1) / rows, a.n * groups), block(256);
   switch (a.cout / 16) {
     case 2: hipLaunchKernelGGL(conv3x3_wgrad_kernel<2>, grid, block, 0, s, a); break;
     case 4: hipLaunchKernelGGL(conv3x3_wgrad_kernel<4>, grid, block, 0, s, a); break;
-    case 8: hipLaunchKernelGGL(conv3x3_wgrad_kernel<8>, grid, block, 0, s, a); break;
-    default: return hipErrorInvalidValue;
+    default: hipLaunchKernelGGL(conv3x3_wgrad_kernel<8>, grid, block, 0, s, a); break;
   }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int slots = (int)(blocks / groups), per4 = (int)(per / 4);
+  const size_t total = (size_t)groups * per4;
+  const f32x4 *part = (const f32x4 *)a.part;
+  const int cin = a.ca + a.cb;
+  auto launch = [&](auto kern, int S) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)((total + 256 / S - 1) / (256 / S))), dim3(256), 0, s, part, slots, per4,
+                       groups, a.cout, cin, a.dw);
+  };
+  // about 16 loads per thread
+  if (slots <= 128) launch(wgrad_reduce_kernel<8>, 8);
+  else if (slots <= 256) launch(wgrad_reduce_kernel<16>, 16);
+  else if (slots <= 512) launch(wgrad_reduce_kernel<32>, 32);
+  else launch(wgrad_reduce_kernel<64>, 64);
   return hipGetLastError();
 }
 
@@ -1025,11 +1120,11 @@ hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const flo
   return hipGetLastError();
 }
 hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
-                    float inv_count, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s) {
+                    float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s) {
   // about 16 pixels per thread: the per-thread partial sums amortise the block's LDS atomics
   const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (npix + 4095) / 4096));
-  hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks), dim3(256), 0, s, d1, out, target, wh, npix, inv_count, dd1, dwh,
-                     dbh, loss);
+  hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks), dim3(256), 0, s, d1, out, target, wh, npix, inv_count, l2, dd1,
+                     dwh, dbh, loss);
   return hipGetLastError();
 }
 hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,

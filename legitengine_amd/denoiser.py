@@ -3,7 +3,7 @@ training loop on the BF16 MFMA U-Net, and evaluation against the
 256-direction FP64 gather.
 
 Pairs: a seeded scene (32 boxes, 8 spheres) seen from a seeded camera pose
-at 1920x1080; a 256x256 crop (origin on the 4-pixel interleave grid, so the
+at 1920x1080; a 256x256 crop (origin on the 8-pixel grid, so the interleave
 pattern phase matches full frames) of the 4-direction gather is the input,
 the 256-direction gather accumulated in FP64 on the same crop the target.
 The input gather marches with the step growth and mip lod of a larger
@@ -31,6 +31,7 @@ CROP = 256
 NOISY_DIRS, TARGET_DIRS, PLAIN_DIRS, STEPS = 4, 256, 32, 64
 NOISY_FOOTPRINT = 256  # the input marches with the target's step growth and lod (256 directions)
 NOISY_STEPS = 0  # and the reference step rule without a cap (the far field matters most)
+NOISY_GRID = 4  # interleave pattern of the input gather (grid^2 direction offsets)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_1080p.safetensors")
 # (logical cin, cout, resolution divisor) of the ten 3x3 convolutions; the
 # 1x1 head maps 32 channels to 3
@@ -55,11 +56,11 @@ class FrameGen:
         self.out32 = dev.DImage(w, h, lgx.RGBA32F)
         self.out64 = dev.DImage(w, h, lgx.RGBA64F)
 
-    def gather(self, dirs, steps=STEPS, precision=lgx.PREC_F32, footprint=0):
-        key = (dirs, steps, precision, footprint)
+    def gather(self, dirs, steps=STEPS, precision=lgx.PREC_F32, footprint=0, grid=4):
+        key = (dirs, steps, precision, footprint, grid)
         if key not in self.gathers:
-            self.gathers[key] = dev.Gather(self.w, self.h, dirs, steps, lgx.SAMPLER_HIZ_POINT, precision=precision,
-                                           footprint_dirs=footprint)
+            self.gathers[key] = dev.Gather(self.w, self.h, dirs, steps, lgx.SAMPLER_HIZ_POINT, grid=grid,
+                                           precision=precision, footprint_dirs=footprint)
         return self.gathers[key]
 
     def view(self, scene_seed, pose=0, boxes=32, spheres=8):
@@ -85,9 +86,11 @@ class FrameGen:
 
 
 def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx.PREC_F32_ACC64,
-                  footprints=(NOISY_FOOTPRINT,)):
+                  footprints=(NOISY_FOOTPRINT,), grid=NOISY_GRID):
     """Inputs X[f] [n, CROP, CROP, 12] for each footprint f and targets T
-    [n, CROP, CROP, 4], all on the GPU."""
+    [n, CROP, CROP, 4], all on the GPU.  The input gather uses a grid x grid
+    interleave pattern; crop origins are multiples of 8 pixels, so the
+    pattern phase matches full frames for grid 4 and 8."""
     fg = FrameGen()
     n = n_scenes * poses
     X = {f: torch.empty((n, CROP, CROP, 12), dtype=torch.float32, device="cuda") for f in footprints}
@@ -99,11 +102,12 @@ def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx
     for i in range(n):
         s, p = seed0 + i // poses, i % poses
         view, proj = fg.view(s, p)
-        x0 = int(rng.integers(0, (W - CROP) // 4 + 1)) * 4
-        y0 = int(rng.integers(0, (H - CROP) // 4 + 1)) * 4
+        x0 = int(rng.integers(0, (W - CROP) // 8 + 1)) * 8
+        y0 = int(rng.integers(0, (H - CROP) // 8 + 1)) * 8
         roi = (x0, y0, CROP, CROP)
         for f in footprints:
-            fg.gather(NOISY_DIRS, NOISY_STEPS, footprint=f)(view, proj, fg.g, fg.pyr, out=fg.out32, roi=roi)
+            fg.gather(NOISY_DIRS, NOISY_STEPS, footprint=f, grid=grid)(view, proj, fg.g, fg.pyr, out=fg.out32,
+                                                                      roi=roi)
             X[f][i] = fg.records(fg.out32, y0, x0, CROP, CROP)
         target_g(view, proj, fg.g, fg.pyr, out=fg.out64, roi=roi)
         T[i] = fg.out64.t[y0:y0 + CROP, x0:x0 + CROP].float()
@@ -148,7 +152,7 @@ def rel_l2(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
-def evaluate(net, seeds, log=print, plain_dirs=(4, 8, 16, 32), footprint=NOISY_FOOTPRINT):
+def evaluate(net, seeds, log=print, plain_dirs=(4, 8, 16, 32), footprint=NOISY_FOOTPRINT, grid=NOISY_GRID):
     """rel-L2 of the denoised 4-direction path and of plain paths against the
     256-direction FP64 gather on full 1080p frames."""
     fg = FrameGen()
@@ -158,7 +162,7 @@ def evaluate(net, seeds, log=print, plain_dirs=(4, 8, 16, 32), footprint=NOISY_F
         fg.gather(TARGET_DIRS, 0, lgx.PREC_F64)(view, proj, fg.g, fg.pyr, out=fg.out64)
         tgt = fg.out64.t.clone()
         row = {"seed": s}
-        fg.gather(NOISY_DIRS, NOISY_STEPS, footprint=footprint)(view, proj, fg.g, fg.pyr, out=fg.out32)
+        fg.gather(NOISY_DIRS, NOISY_STEPS, footprint=footprint, grid=grid)(view, proj, fg.g, fg.pyr, out=fg.out32)
         row["noisy4"] = rel_l2(fg.out32.t, tgt)
         den = net.denoise(fg.g, fg.out32)
         row["denoised4"] = rel_l2(den.t, tgt)
@@ -190,6 +194,7 @@ def main(argv=None):
     t.add_argument("--report", default=None)
     t.add_argument("--footprint", type=int, default=NOISY_FOOTPRINT)
     t.add_argument("--schedule", default="cosine", choices=("cosine", "const"))
+    t.add_argument("--loss", default="l1", choices=("l1", "l2"))
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", default="200,201,202,203,204,205,206,207")
@@ -206,6 +211,7 @@ def main(argv=None):
         net = dev.UNet(args.batch, CROP, CROP, seed=0)
         if args.resume:
             net.load(args.resume)
+        net.set_loss(args.loss)
         net.autotune(args.batch, CROP, CROP, backward=True)
         hist, secs = train(net, X, T, args.steps, args.batch, args.lr, log=log, schedule=args.schedule)
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
@@ -220,7 +226,8 @@ def main(argv=None):
             with open(args.report, "w") as f:
                 json.dump({"history": hist, "train_seconds": secs, "steps": args.steps, "batch": args.batch,
                            "scenes": args.scenes, "poses": args.poses, "footprint": args.footprint,
-                           "schedule": args.schedule, "eval": rows}, f)
+                           "schedule": args.schedule, "loss": args.loss, "lr": args.lr,
+                           "resume": args.resume, "eval": rows}, f)
     else:
         net = dev.UNet(1, H, W)
         net.load(args.weights)

@@ -294,6 +294,11 @@ class UNet:
         lgx.check(lgx.lib().lgx_unet_backward(self.h, C.c_void_p(x12.data_ptr()), C.c_void_p(target.data_ptr()), n, h,
                                               w, lp, stream_handle(stream)), "lgx_unet_backward")
 
+    def set_loss(self, kind):
+        """Training loss of later backward calls: "l1" (mean absolute error) or "l2" (mean squared error)."""
+        code = {"l1": lgx.UNET_LOSS_L1, "l2": lgx.UNET_LOSS_L2}[kind]
+        lgx.check(lgx.lib().lgx_unet_set_loss(self.h, code), "lgx_unet_set_loss")
+
     def adam_step(self, lr, stream=None):
         lgx.check(lgx.lib().lgx_unet_adam_step(self.h, C.c_float(lr), stream_handle(stream)), "lgx_unet_adam_step")
 

@@ -19,6 +19,7 @@ MIP_AVG, MIP_MAX = 0, 1
 SAMPLER_TRILINEAR, SAMPLER_HIZ_POINT = 0, 1
 PREC_F32, PREC_F64, PREC_F32_ACC64 = 0, 1, 2
 OUT_INTERLEAVED, OUT_SLICES = 0, 1
+UNET_LOSS_L1, UNET_LOSS_L2 = 0, 1
 OK, EINVAL, EFORMAT, ESIZE, EUNSUPPORTED, EWORKSPACE, ENODEVICE = 0, -1, -2, -3, -4, -5, -6
 
 
@@ -118,6 +119,7 @@ SIGNATURES = {
     "lgx_unet_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                     C.c_void_p]),
     "lgx_unet_adam_step": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
+    "lgx_unet_set_loss": (C.c_int, [C.c_void_p, C.c_int32]),
     "lgx_unet_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                       C.c_void_p, C.c_void_p]),
     "lgx_unet_output": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),

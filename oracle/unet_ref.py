@@ -3,7 +3,7 @@
 The network has no counterpart in the reference (it replaces the regression
 DenoiserPass, SSVGIRenderer.h:265-302 / denoiser.frag:112-184); this numpy
 version is the checker of the BF16 MFMA kernels: forward, analytic backward
-of the mean-L1 loss, and Adam.  Its analytic gradient is pinned by FP64
+of the mean-L1 or mean-squared loss, and Adam.  Its analytic gradient is pinned by FP64
 central finite differences (tests/test_unet_oracle.py).
 
 Parameter layout (flat, FP32 on the device): for each of the 10 convolutions
@@ -127,18 +127,19 @@ def forward(flat, x12, keep=False):
     return (out, a) if keep else out
 
 
-def loss(flat, x12, target):
-    out = forward(flat, x12)
-    return float(np.abs(out - np.asarray(target, np.float64)[..., :3]).mean())
+def loss(flat, x12, target, kind="l1"):
+    d = forward(flat, x12) - np.asarray(target, np.float64)[..., :3]
+    return float((d * d).mean() if kind == "l2" else np.abs(d).mean())
 
 
-def backward(flat, x12, target):
-    """Mean-L1 loss and its gradient w.r.t. the flat parameters."""
+def backward(flat, x12, target, kind="l1"):
+    """Mean-L1 (kind "l1") or mean-squared ("l2") loss over rgb and its
+    gradient w.r.t. the flat parameters."""
     L, wh, bh = unpack(flat)
     out, a = forward(flat, x12, keep=True)
     t = np.asarray(target, np.float64)[..., :3]
     cnt = out.size
-    g = np.sign(out - t) / cnt
+    g = (2.0 * (out - t) if kind == "l2" else np.sign(out - t)) / cnt
     grads = [None] * 10
     dwh = g.reshape(-1, 3).T @ a["d1"].reshape(-1, 32)
     dbh = g.reshape(-1, 3).sum(0)
@@ -170,7 +171,8 @@ def backward(flat, x12, target):
     dx = step(1, a["e1a"], de1)
     de1a = dx * (a["e1a"] > 0)
     step(0, a["x"], de1a)
-    lval = float(np.abs(out - t).mean())
+    d = out - t
+    lval = float((d * d).mean() if kind == "l2" else np.abs(d).mean())
     return lval, pack(grads, dwh, dbh)
 
 

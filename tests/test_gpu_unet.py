@@ -66,15 +66,17 @@ def test_forward_matches_oracle(dev):
     assert rel(o[..., :3] - x12[..., :3], ref - x12[..., :3]) <= 3e-2
 
 
-def test_backward_matches_oracle(dev):
+@pytest.mark.parametrize("kind", ["l1", "l2"])
+def test_backward_matches_oracle(dev, kind):
     flat, x12, target = case(2)
     net = dev.UNet(2, 32, 32)
     net.set(0, flat)
+    net.set_loss(kind)
     loss = torch.zeros(1, device="cuda")
     net.backward(torch.from_numpy(x12).cuda(), torch.from_numpy(target).cuda(), loss)
     torch.cuda.synchronize()
     g, _ = net.get(1)
-    l_ref, g_ref = U.backward(flat.astype(np.float64), x12, target)
+    l_ref, g_ref = U.backward(flat.astype(np.float64), x12, target, kind)
     assert abs(loss.item() - l_ref) <= 1e-2 * l_ref
     assert rel(g, g_ref) <= 5e-2
     off = 0
@@ -152,8 +154,10 @@ def test_denoise_from_gbuffer_matches_records(dev):
 def test_register_tiles_bit_identical_and_autotune(dev):
     """Every convolution register tile accumulates each output in the same
     order, so forward outputs are bit-identical whatever the tiles, and so
-    are the activation gradients; the weight gradients differ only by the
-    order of their FP32 atomic sums.  The autotuner only picks among them."""
+    are the activation gradients and the convolution weight gradients (the
+    per-workgroup partial sums are reduced in a fixed order); bias and head
+    gradients differ only by the order of their FP32 atomic sums.  The
+    autotuner only picks among them."""
     from legitengine_amd import lgx
     flat, x12, target = case(5, n=2, h=48, w=80)
     net = dev.UNet(2, 48, 80)
@@ -164,6 +168,13 @@ def test_register_tiles_bit_identical_and_autotune(dev):
     ref_out = net.forward(x).cpu().numpy()
     net.backward(x, t)
     ref_grad = net.get(1)[0]
+    wmask = np.zeros(ref_grad.size, bool)
+    off = 0
+    for ci, co in U.LAYERS:
+        wmask[off:off + co * ci * 9] = True
+        off += co * ci * 9 + co
+    net.backward(x, t)
+    assert net.get(1)[0][wmask].tobytes() == ref_grad[wmask].tobytes()  # same call twice
     counts_f = [lgx.lib().lgx_unet_conv_tiles(c) for c in couts]
     counts_d = [lgx.lib().lgx_unet_conv_tiles(c) for c in cins]
     assert min(counts_f) >= 2 and min(counts_d) >= 2
@@ -171,7 +182,9 @@ def test_register_tiles_bit_identical_and_autotune(dev):
         net.set_tiles([k % c for c in counts_f], [k % c for c in counts_d])
         assert net.forward(x).cpu().numpy().tobytes() == ref_out.tobytes(), k
         net.backward(x, t)
-        assert rel(net.get(1)[0], ref_grad) < 1e-5, k
+        g = net.get(1)[0]
+        assert g[wmask].tobytes() == ref_grad[wmask].tobytes(), k
+        assert rel(g, ref_grad) < 1e-5, k
     rep = net.autotune(2, 48, 80, backward=True)
     f, d = net.tiles()
     assert [r["best"] for r in rep["forward"]] == f

@@ -1,6 +1,6 @@
 """Pins the FP64 U-Net restatement (oracle/unet_ref.py) independently of its
-own backward formulas: FP64 central finite differences of the mean-L1 loss
-on a 16x16 crop, per-layer directional derivatives and sampled coordinates;
+own backward formulas: FP64 central finite differences of the mean-L1 and
+mean-squared losses on a 16x16 crop, per-layer directional derivatives and sampled coordinates;
 closed forms for the building blocks (convolution against a direct
 summation, pooling/upsampling adjointness)."""
 import numpy as np
@@ -73,11 +73,11 @@ def layer_slices():
     return out
 
 
-@pytest.mark.parametrize("seed", [3, 4])
-def test_backward_matches_finite_differences(seed):
+@pytest.mark.parametrize("seed,kind", [(3, "l1"), (4, "l1"), (5, "l2")])
+def test_backward_matches_finite_differences(seed, kind):
     flat, x12, target = random_case(seed)
-    l0, grad = U.backward(flat, x12, target)
-    assert np.isclose(l0, U.loss(flat, x12, target))
+    l0, grad = U.backward(flat, x12, target, kind)
+    assert np.isclose(l0, U.loss(flat, x12, target, kind))
     rng = np.random.default_rng(seed + 100)
     # small enough that no ReLU or |.| kink is crossed along the direction,
     # large enough that FP64 rounding (~1e-16 / eps) stays far below the bar
@@ -86,14 +86,14 @@ def test_backward_matches_finite_differences(seed):
     for (a, b) in layer_slices():
         v = np.zeros_like(flat)
         v[a:b] = rng.normal(size=b - a)
-        fd = (U.loss(flat + eps * v, x12, target) - U.loss(flat - eps * v, x12, target)) / (2 * eps)
+        fd = (U.loss(flat + eps * v, x12, target, kind) - U.loss(flat - eps * v, x12, target, kind)) / (2 * eps)
         an = grad @ v
         assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)) + 1e-9, (a, fd, an)
     # sampled coordinates across the whole vector
     for i in rng.choice(flat.size, 24, replace=False):
         e = np.zeros_like(flat)
         e[i] = 1.0
-        fd = (U.loss(flat + eps * e, x12, target) - U.loss(flat - eps * e, x12, target)) / (2 * eps)
+        fd = (U.loss(flat + eps * e, x12, target, kind) - U.loss(flat - eps * e, x12, target, kind)) / (2 * eps)
         assert abs(fd - grad[i]) <= 1e-6 + 1e-4 * abs(grad[i]), (i, fd, grad[i])
 
 

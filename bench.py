@@ -258,8 +258,9 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     net.load(path)
     unet_tune = net.autotune(1, H, W)
     pyr0 = dev.PyramidDev(g, LEVELS, sampler).build()
-    den_tile, _ = dev.autotune_gather(g, pyr0, view, proj, DEN_DIRS, dn.NOISY_STEPS, sampler, dn.NOISY_FOOTPRINT)
-    rden = dev.Renderer(W, H, DEN_DIRS, dn.NOISY_STEPS, LEVELS, sampler, GRID, denoiser=lgx.DENOISER_UNET,
+    den_tile, _ = dev.autotune_gather(g, pyr0, view, proj, DEN_DIRS, dn.NOISY_STEPS, sampler, dn.NOISY_FOOTPRINT,
+                                      grid=dn.NOISY_GRID)
+    rden = dev.Renderer(W, H, DEN_DIRS, dn.NOISY_STEPS, LEVELS, sampler, dn.NOISY_GRID, denoiser=lgx.DENOISER_UNET,
                         kind="mixed", profile=True, footprint_dirs=dn.NOISY_FOOTPRINT, tile=den_tile)
     rden.attach_unet(net)
     den_ms, den_passes = time_renderer(rden, g, view, proj, args.steps, args.warmup)
@@ -320,7 +321,8 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         e["renderer"].close()
     return {
         "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather (step growth and lod of {dn.NOISY_FOOTPRINT} "
-                    f"directions, reference step rule without cap), {LEVELS}-level Hi-Z, 4x4 interleave, BF16 U-Net "
+                    f"directions, reference step rule without cap), {LEVELS}-level Hi-Z, "
+                    f"{dn.NOISY_GRID}x{dn.NOISY_GRID} interleave, BF16 U-Net "
                     "denoiser pass (3 levels, 32/64/128 channels, FP32 accumulation)",
         **timing,
         "rel_l2_vs_fp64_256": round(err, 5),
@@ -398,7 +400,7 @@ def batch_leg(args, sampler):
     path = args.weights or dn.WEIGHTS
     net.load(path)
     net.autotune(1, 1080, 1920)  # register tiles at the most common extent
-    b = dev.Batch(views, dirs=DEN_DIRS, max_steps=dn.NOISY_STEPS, levels=LEVELS, sampler=sampler, grid=GRID,
+    b = dev.Batch(views, dirs=DEN_DIRS, max_steps=dn.NOISY_STEPS, levels=LEVELS, sampler=sampler, grid=dn.NOISY_GRID,
                   footprint_dirs=dn.NOISY_FOOTPRINT, net=net)
     log(f"batch of {len(views)} views created")
     tune = b.autotune()

@@ -31,7 +31,7 @@ CROP = 256
 NOISY_DIRS, TARGET_DIRS, PLAIN_DIRS, STEPS = 4, 256, 32, 64
 NOISY_FOOTPRINT = 256  # the input marches with the target's step growth and lod (256 directions)
 NOISY_STEPS = 0  # and the reference step rule without a cap (the far field matters most)
-NOISY_GRID = 4  # interleave pattern of the input gather (grid^2 direction offsets)
+NOISY_GRID = 8  # interleave pattern of the input gather: 4 directions x 64 offsets = 256 angles per 8x8 block
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_1080p.safetensors")
 # (logical cin, cout, resolution divisor) of the ten 3x3 convolutions; the
 # 1x1 head maps 32 channels to 3
@@ -195,10 +195,12 @@ def main(argv=None):
     t.add_argument("--footprint", type=int, default=NOISY_FOOTPRINT)
     t.add_argument("--schedule", default="cosine", choices=("cosine", "const"))
     t.add_argument("--loss", default="l1", choices=("l1", "l2"))
+    t.add_argument("--grid", type=int, default=NOISY_GRID)
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", default="200,201,202,203,204,205,206,207")
     e.add_argument("--footprint", type=int, default=NOISY_FOOTPRINT)
+    e.add_argument("--grid", type=int, default=NOISY_GRID)
     args = ap.parse_args(argv)
     dev.require_device()
 
@@ -206,7 +208,7 @@ def main(argv=None):
         print(msg, flush=True)
 
     if args.cmd == "train":
-        X, T = build_dataset(args.scenes, args.poses, log=log, footprints=(args.footprint,))
+        X, T = build_dataset(args.scenes, args.poses, log=log, footprints=(args.footprint,), grid=args.grid)
         X = X[args.footprint]
         net = dev.UNet(args.batch, CROP, CROP, seed=0)
         if args.resume:
@@ -221,17 +223,18 @@ def main(argv=None):
         infer = dev.UNet(1, H, W)
         infer.load(args.out)
         infer.autotune(1, H, W)
-        rows = evaluate(infer, [int(s) for s in args.eval_seeds.split(",") if s], footprint=args.footprint)
+        rows = evaluate(infer, [int(s) for s in args.eval_seeds.split(",") if s], footprint=args.footprint,
+                        grid=args.grid)
         if args.report:
             with open(args.report, "w") as f:
                 json.dump({"history": hist, "train_seconds": secs, "steps": args.steps, "batch": args.batch,
                            "scenes": args.scenes, "poses": args.poses, "footprint": args.footprint,
-                           "schedule": args.schedule, "loss": args.loss, "lr": args.lr,
+                           "schedule": args.schedule, "loss": args.loss, "lr": args.lr, "grid": args.grid,
                            "resume": args.resume, "eval": rows}, f)
     else:
         net = dev.UNet(1, H, W)
         net.load(args.weights)
-        evaluate(net, [int(s) for s in args.seeds.split(",") if s], footprint=args.footprint)
+        evaluate(net, [int(s) for s in args.seeds.split(",") if s], footprint=args.footprint, grid=args.grid)
     return 0
 
 

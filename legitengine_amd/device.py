@@ -160,14 +160,15 @@ class Gather:
         return out
 
 
-def autotune_gather(gb, pyr, view, proj, dirs, max_steps, sampler=lgx.SAMPLER_HIZ_POINT, footprint_dirs=0, reps=3):
+def autotune_gather(gb, pyr, view, proj, dirs, max_steps, sampler=lgx.SAMPLER_HIZ_POINT, footprint_dirs=0, reps=3,
+                    grid=4):
     """Times every gather launch shape on this frame (hipEvents on the current
     stream) and returns (fastest tile, {tile: ms}); all shapes give
     bit-identical results."""
     out = DImage(gb.w, gb.h, lgx.RGBA32F)
     times = {}
     for t in range(lgx.lib().lgx_gather_tiles()):
-        g = Gather(gb.w, gb.h, dirs, max_steps, sampler, tile=t, footprint_dirs=footprint_dirs)
+        g = Gather(gb.w, gb.h, dirs, max_steps, sampler, grid=grid, tile=t, footprint_dirs=footprint_dirs)
         g(view, proj, gb, pyr, out=out)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()

@@ -133,20 +133,21 @@ def test_gather_c1(dev, c1, seed_idx, sampler, dirs, steps):
     assert np.array_equal(counts, ref_steps)
 
 
-@pytest.mark.parametrize("footprint", [32, 64])
-def test_gather_footprint(dev, c1, footprint):
-    """4 directions marching with a 32- or 64-direction footprint (the
-    denoiser's input) against the oracle with the same footprint."""
+@pytest.mark.parametrize("footprint,steps,grid", [(32, 64, 4), (64, 64, 4), (256, 0, 4), (256, 0, 8)])
+def test_gather_footprint(dev, c1, footprint, steps, grid):
+    """4 directions marching with a larger direction count's footprint, with
+    or without a step cap, on a 4x4 or 8x8 interleave pattern (the
+    denoiser's input), against the oracle with the same parameters."""
     sc, view, proj, gb = c1[1]
     g = dev.GBufferDev.from_host(gb)
     pyr = dev.PyramidDev(g, 5, lgx.SAMPLER_HIZ_POINT).build()
     counts = torch.zeros((256, 256), dtype=torch.int32, device="cuda")
-    out = dev.Gather(256, 256, 4, 64, lgx.SAMPLER_HIZ_POINT, footprint_dirs=footprint)(view, proj, g, pyr,
-                                                                                       step_counts=counts)
+    out = dev.Gather(256, 256, 4, steps, lgx.SAMPLER_HIZ_POINT, grid=grid, footprint_dirs=footprint)(
+        view, proj, g, pyr, step_counts=counts)
     torch.cuda.synchronize()
     dep_b, rad_b, _, _ = _pyr_host(gb, 5, lgx.SAMPLER_HIZ_POINT)
-    ref, ref_steps = orc.gather(view, proj, gb["depth"], orc.widen(gb["normal"]), dep_b, rad_b, 4, 64,
-                                lgx.SAMPLER_HIZ_POINT, with_steps=True, footprint_dirs=footprint)
+    ref, ref_steps = orc.gather(view, proj, gb["depth"], orc.widen(gb["normal"]), dep_b, rad_b, 4, steps,
+                                lgx.SAMPLER_HIZ_POINT, grid=grid, with_steps=True, footprint_dirs=footprint)
     err = orc.rel_l2(out.numpy(), ref)
     assert err <= GATHER_TOL, err
     assert np.array_equal(counts.cpu().numpy(), ref_steps)

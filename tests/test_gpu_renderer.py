@@ -114,15 +114,15 @@ def test_renderer_rejects_bad_config(dev):
     assert not h.value
 
 
-@pytest.mark.parametrize("footprint", [0, 32])
-def test_unet_denoiser_pass(dev, footprint):
+@pytest.mark.parametrize("footprint,steps,grid", [(0, 64, 4), (32, 64, 4), (256, 0, 8)])
+def test_unet_denoiser_pass(dev, footprint, steps, grid):
     """LGX_DENOISER_UNET: the renderer's DenoiserPass equals lgx_unet_denoise
     run on the renderer's own deinterleaved gather output, bit for bit, and
-    the gather honours footprint_dirs."""
+    the gather honours footprint_dirs, the step cap and the pattern size."""
     w, h = 320, 192
     view, proj, gb, g = frame_inputs(dev, 23, w, h, "mixed")
     net = dev.UNet(1, h, w, seed=5)
-    r = dev.Renderer(w, h, 4, 64, 6, lgx.SAMPLER_HIZ_POINT, kind="mixed", denoiser=lgx.DENOISER_UNET,
+    r = dev.Renderer(w, h, 4, steps, 6, lgx.SAMPLER_HIZ_POINT, grid, kind="mixed", denoiser=lgx.DENOISER_UNET,
                      validate=True, footprint_dirs=footprint)
     with pytest.raises(lgx.LgxError):
         r.render(g, view, proj)  # no network attached yet
@@ -131,9 +131,9 @@ def test_unet_denoiser_pass(dev, footprint):
     torch.cuda.synchronize()
     ind = r.intermediate("indirect")
     pyr = dev.PyramidDev(g, 6, lgx.SAMPLER_HIZ_POINT).build()
-    sl = dev.Gather(w, h, 4, 64, lgx.SAMPLER_HIZ_POINT, out_layout=lgx.OUT_SLICES,
+    sl = dev.Gather(w, h, 4, steps, lgx.SAMPLER_HIZ_POINT, grid=grid, out_layout=lgx.OUT_SLICES,
                     footprint_dirs=footprint)(view, proj, g, pyr)
-    assert dev.permute(sl, inverse=True).numpy().tobytes() == ind.tobytes()
+    assert dev.permute(sl, inverse=True, gx=grid, gy=grid).numpy().tobytes() == ind.tobytes()
     noisy = dev.DImage.from_numpy(ind, lgx.RGBA32F)
     ref = net.denoise(g, noisy).numpy()
     assert r.output().tobytes() == ref.tobytes()

@@ -745,20 +745,25 @@ __global__ __launch_bounds__(256) void head_fwd_img_kernel(const bf16 *d1, const
 }
 
 // Loss head backward.  Mean L1: g = sign(out - target) / count per channel;
-// mean squared error (l2): g = 2 (out - target) / count.  dW_h, db_h
-// accumulated per workgroup then atomically; dd1 = mask(d1 > 0) W_h^T g.
+// mean squared error (l2): g = 2 (out - target) / count.  dd1 = mask(d1 > 0)
+// W_h^T g.  Four threads share a pixel, each owning 8 of its 32 channels
+// (one 16-byte load and store), so a thread keeps 3 x 8 head-weight partial
+// sums; they are reduced over the lanes of a wave with shuffles, over the
+// waves in LDS, and added to dW_h, db_h once per workgroup.
 __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const float4 *out, const float4 *target,
                                                        const float *wh, size_t npix, float inv_count, int l2,
                                                        bf16 *dd1, float *dwh, float *dbh, float *loss) {
-  // each thread walks pixels p, p + stride, ... keeping its head-gradient
-  // partial sums in registers, then adds them once to the block's LDS sums
-  __shared__ float sw[96 + 3 + 1];
-  for (int i = threadIdx.x; i < 100; i += 256) sw[i] = 0.f;
-  __syncthreads();
-  float aw[96], ab[3] = {0.f, 0.f, 0.f}, al = 0.f;
+  __shared__ float red[4][100];
+  const int q = threadIdx.x & 3, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float w8[3][8], aw[3][8], ab[3] = {0.f, 0.f, 0.f}, al = 0.f;
 #pragma unroll
-  for (int i = 0; i < 96; i++) aw[i] = 0.f;
-  for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < npix; p += (size_t)gridDim.x * 256) {
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      w8[c][i] = wh[c * 32 + q * 8 + i];
+      aw[c][i] = 0.f;
+    }
+  for (size_t p = ((size_t)blockIdx.x * 256 + threadIdx.x) >> 2; p < npix; p += (size_t)gridDim.x * 64) {
     const float4 o = out[p], t = target[p];
     const float df[3] = {o.x - t.x, o.y - t.y, o.z - t.z};
     float g[3];
@@ -768,34 +773,50 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const flo
       al += l2 ? df[c] * df[c] : fabsf(df[c]);
       ab[c] += g[c];
     }
+    const bf16x8 v = *(const bf16x8 *)(d1 + p * 32 + q * 8);
+    bf16x8 o8;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const bf16x8 v = *(const bf16x8 *)(d1 + p * 32 + q * 8);
-      bf16x8 o8;
+    for (int i = 0; i < 8; i++) {
+      const float f = bf2f(v[i]);
+      float sacc = 0.f;
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        const int k = q * 8 + i;
-        const float f = bf2f(v[i]);
-        float sacc = 0.f;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          sacc += wh[c * 32 + k] * g[c];
-          aw[c * 32 + k] += g[c] * f;
-        }
-        o8[i] = f > 0.f ? f2bf(sacc) : f2bf(0.f);
+      for (int c = 0; c < 3; c++) {
+        sacc += w8[c][i] * g[c];
+        aw[c][i] += g[c] * f;
       }
-      *(bf16x8 *)(dd1 + p * 32 + q * 8) = o8;
+      o8[i] = f > 0.f ? f2bf(sacc) : f2bf(0.f);
     }
+    *(bf16x8 *)(dd1 + p * 32 + q * 8) = o8;
   }
+  // lanes with the same q hold the same channels: reduce over lane bits 2..5
 #pragma unroll
-  for (int i = 0; i < 96; i++) atomicAdd(&sw[i], aw[i]);
+  for (int m = 4; m < 64; m <<= 1) {
 #pragma unroll
-  for (int c = 0; c < 3; c++) atomicAdd(&sw[96 + c], ab[c]);
-  atomicAdd(&sw[99], al);
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) aw[c][i] += __shfl_xor(aw[c][i], m);
+      ab[c] += __shfl_xor(ab[c], m);
+    }
+    al += __shfl_xor(al, m);
+  }
+  if (lane < 4) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int i = 0; i < 8; i++) red[wave][c * 32 + q * 8 + i] = aw[c][i];
+  }
+  if (lane == 0) {  // every q saw the same pixels: the q = 0 lanes' sums count once
+#pragma unroll
+    for (int c = 0; c < 3; c++) red[wave][96 + c] = ab[c];
+    red[wave][99] = al;
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < 96; i += 256) atomicAdd(&dwh[i], sw[i]);
-  if (threadIdx.x < 3) atomicAdd(&dbh[threadIdx.x], sw[96 + threadIdx.x]);
-  if (threadIdx.x == 0 && loss) atomicAdd(loss, sw[99] * inv_count);
+  if (threadIdx.x < 100) {
+    const float v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    if (threadIdx.x < 96) atomicAdd(&dwh[threadIdx.x], v);
+    else if (threadIdx.x < 99) atomicAdd(&dbh[threadIdx.x - 96], v);
+    else if (loss) atomicAdd(loss, v * inv_count);
+  }
 }
 
 // Adam on FP32 master parameters (Kingma & Ba 2015, bias-corrected).
@@ -1121,8 +1142,8 @@ hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const flo
 }
 hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
                     float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s) {
-  // about 16 pixels per thread: the per-thread partial sums amortise the block's LDS atomics
-  const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (npix + 4095) / 4096));
+  // 64 pixels per workgroup pass; up to 512 workgroups, so 512 atomic sums per head parameter
+  const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(512, (npix + 63) / 64));
   hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks), dim3(256), 0, s, d1, out, target, wh, npix, inv_count, l2, dd1,
                      dwh, dbh, loss);
   return hipGetLastError();

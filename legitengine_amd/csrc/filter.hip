@@ -50,15 +50,16 @@ template <int NFMT> __global__ __launch_bounds__(256) void bilateral_kernel(cons
   const int px = blockIdx.x * 16 + tx, py = blockIdx.y * 16 + ty;
   if (px >= a.w || py >= a.h) return;
   const float4 gc = s_geo[(ty + r) * kBilatSpan + tx + r];
+  // exp(-|dd| ds) exp(-|dn|^2 ns) = exp2(-(|dd| ds + |dn|^2 ns) log2(e)): one
+  // hardware exp2 (1 ulp) per tap instead of two library expf
+  const float kd = a.ds * 1.44269504f, kn = a.ns * 1.44269504f;
   float c0 = 0.f, c1 = 0.f, c2 = 0.f, sw = 0.f;
   for (int y = -r; y < r; y++)
     for (int x = -r; x < r; x++) {
       const int li = (ty + r + y) * kBilatSpan + tx + r + x;
       const float4 gs = s_geo[li];
       const float dx = gs.y - gc.y, dy = gs.z - gc.z, dz = gs.w - gc.w;
-      float wgt = 1.0f;
-      wgt *= expf(-fabsf(gs.x - gc.x) * a.ds);
-      wgt *= expf(-(dx * dx + dy * dy + dz * dz) * a.ns);
+      const float wgt = __builtin_amdgcn_exp2f(-(fabsf(gs.x - gc.x) * kd + (dx * dx + dy * dy + dz * dz) * kn));
       const float4 c = s_col[li];
       c0 += c.x * wgt;
       c1 += c.y * wgt;

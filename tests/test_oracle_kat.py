@@ -365,6 +365,68 @@ def test_gather_plane_no_hits_and_ambient_at_axis():
     assert 3.0 < e64 / e256 < 5.0
 
 
+def test_hammersley_64_exact():
+    """The 8x8 pattern's 64 offsets: (k / 64, bitreverse6(k) / 64) exactly."""
+    import ctypes as C
+    out = (C.c_float * 2)()
+    for k in range(64):
+        orc.lib().orc_hammersley(k, 64, out)
+        assert out[0] == k / 64.0
+        assert out[1] == int(f"{k:06b}"[::-1], 2) / 64.0
+
+
+def test_gather_plane_grid8():
+    """The planar known answer holds on the 8x8 pattern as on the 4x4 one."""
+    view, P, depth, octn, rad0 = plane_scene(64, 48, radiance=0.0)
+    _, _, _, _, rad5 = plane_scene(64, 48, radiance=5.0)
+    o0, s0 = orc.gather(view, P, depth, octn, [depth], [rad0], 8, 0, 0, grid=8, precision="f64", with_steps=True)
+    o5, _ = orc.gather(view, P, depth, octn, [depth], [rad5], 8, 0, 0, grid=8, precision="f64", with_steps=True)
+    assert np.allclose(o0, o5, rtol=0, atol=1e-6)
+    assert (s0 > 0).all()
+    assert abs(o0[24, 32, 0] / 0.01 - 1.0) < 3e-2
+
+
+@pytest.mark.parametrize("grid", [4, 8])
+def test_pattern_index_and_direction(grid):
+    """Pins the pattern index (x % g) + (y % g) g and the per-pattern screen
+    direction 2 pi (k / g^2) / D (pi = 3.1415f, :169) through the executed
+    step counts of a one-direction gather, recomputed independently in
+    float64: the count is the number of offsets off_j = near ratio^(j + lin)
+    + 1 - near (:212-217) whose sample stays strictly inside the screen.  A
+    transposed index or a wrong angle moves most pixels' counts."""
+    w, h = 64, 48
+    view, P, depth, octn, rad = plane_scene(w, h, radiance=1.0)
+    _, steps = orc.gather(view, P, depth, octn, [depth], [rad], 1, 0, 0, grid=grid, precision="f64",
+                          with_steps=True, footprint_dirs=16)
+    pi = float(np.float32(3.1415))
+    g2 = grid * grid
+    near, ratio = w / 1000.0, 2 * pi / 16 + 1
+
+    def count(x, y, kx, ky):
+        k = kx + ky * grid
+        ang = 2 * pi / 1 * (k / g2)
+        lin = int(f"{k:0{g2.bit_length() - 1}b}"[::-1], 2) / g2
+        dx, dy = np.cos(ang), np.sin(ang)
+        cx, cy = x + 0.5, y + 0.5
+        ts = [t for t in ((0 - cx) / dx if dx else np.inf, (w - cx) / dx if dx else np.inf,
+                          (0 - cy) / dy if dy else np.inf, (h - cy) / dy if dy else np.inf)]
+        tmax = min(max(ts[0], ts[1]), max(ts[2], ts[3]))
+        iters = int(np.log(abs(tmax) / near) / np.log(ratio)) + 1
+        n = 0
+        for j in range(iters):
+            off = near * ratio ** (j + lin) + 1 - near
+            sx, sy = (cx + dx * off) / w, (cy + dy * off) / h
+            n += 0 < sx < 1 and 0 < sy < 1
+        return n
+
+    ys, xs = np.mgrid[0:h, 0:w]
+    ref = np.array([[count(x, y, x % grid, y % grid) for x in range(w)] for y in range(h)])
+    assert (ref == steps).mean() > 0.995, (ref == steps).mean()
+    if grid > 1:  # the transposed index is far off
+        bad = np.array([[count(x, y, y % grid, x % grid) for x in range(w)] for y in range(h)])
+        assert (bad == steps).mean() < 0.6
+
+
 def test_gather_linear_in_radiance_and_float_vs_double():
     from legitengine_amd import lgx
     sc = lgx.make_scene(11, 8, 2)

@@ -105,7 +105,7 @@ typedef struct {
   int32_t width, height; /* viewport (viewportSize.xy) */
   int32_t dirs;      /* directions per pixel (reference: 4) */
   int32_t max_steps; /* step cap S; <= 0 = reference step rule only */
-  int32_t grid;      /* interleave pattern size (reference: 4) */
+  int32_t grid;      /* interleave pattern width (reference: 4; square unless grid_y is set) */
   int32_t sampler;   /* LGX_SAMPLER_* */
   int32_t precision; /* LGX_PREC_*: F64 computes and accumulates in double (output RGBA64F) */
   int32_t out_layout;/* LGX_OUT_*: SLICES writes the output deinterleaved (grid x grid slices) */
@@ -115,6 +115,10 @@ typedef struct {
   int32_t footprint_dirs; /* direction count in the step growth and lod (indirectLighting.frag:212,217,234);
                              <= 0 = dirs, as in the shader.  A few directions with a finer footprint feed the
                              learned denoiser. */
+  int32_t grid_y;    /* interleave pattern height, <= 0 = grid (the shader's square pattern).  Pixel (x, y)
+                        takes pattern index (x % grid) + (y % grid_y) * grid of grid * grid_y Hammersley
+                        offsets (:155-164); grid * grid_y <= 64.  LGX_OUT_SLICES needs the pattern to divide
+                        the frame. */
 } lgx_gather_params;
 
 /* Workspace for the gather: per-(pattern, direction) screen directions and
@@ -222,6 +226,8 @@ typedef struct {
   int32_t profile;   /* 1: per-pass hipEvent timings */
   int32_t validate;  /* 1: passes may only touch the views and buffers they declare */
   int32_t footprint_dirs; /* gather step growth / lod direction count (lgx_gather_params), <= 0 = dirs */
+  int32_t grid_y;    /* interleave pattern height (lgx_gather_params), <= 0 = grid.  A pattern that does not
+                        divide the frame has the gather write the image directly (no InterleavePass). */
 } lgx_renderer_config;
 
 int lgx_renderer_create(const lgx_renderer_config *cfg, lgx_renderer **out);
@@ -351,6 +357,7 @@ typedef struct {
   int32_t dirs, max_steps, levels, grid, sampler, footprint_dirs;
   int32_t tile;       /* gather launch shape */
   int32_t workgroups; /* persistent gather workgroups, 0 = one per block */
+  int32_t grid_y;     /* interleave pattern height (lgx_gather_params), <= 0 = grid */
 } lgx_batch_config;
 /* net: created with max n >= 1 and max h x w covering every view; NULL =
  * no denoiser (the views' outputs are the noisy gathers). */

@@ -71,6 +71,7 @@ struct SsvgiConfig {
   int32_t width = 1920, height = 1080;
   int32_t dirs = 32, max_steps = 64, levels = 6, grid = 4;
   int32_t footprint_dirs = 0;  // gather step growth / lod direction count, 0 = dirs
+  int32_t grid_y = 0;          // interleave pattern height, 0 = grid (square)
   int32_t sampler = LGX_SAMPLER_HIZ_POINT;
   int32_t denoiser = LGX_DENOISER_BILATERAL;
   int32_t denoise_passes = 2, denoise_radius = 4;

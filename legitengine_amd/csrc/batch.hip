@@ -88,6 +88,7 @@ struct lgx_batch {
     p.dirs = c.dirs;
     p.max_steps = c.max_steps;
     p.grid = c.grid;
+    p.grid_y = c.grid_y;
     p.sampler = c.sampler;
     p.precision = LGX_PREC_F32;
     p.out_layout = LGX_OUT_INTERLEAVED;

@@ -37,11 +37,11 @@ template <class T> struct GatherArgs {
   T cam[3];
   T e0[3], ex[3], ey[3];  // unnormalised ray through screen point s: e0 + s.x*ex + s.y*ey
   T W, H, near_step, log_ratio, ambient;
-  int w, h, grid, cells_x, cells_y, D, S_tab;
+  int w, h, grid, grid_y, cells_x, cells_y, D, S_tab;  // pattern grid x grid_y
   T rW, rH;        // RN(1/W), RN(1/H) for the exact division by the extent
-  const T *dirs;   // [grid^2][D][2]
-  const T *steps;  // [grid^2][S_tab][2] = (offset, lod)
-  const int *level_start;  // [grid^2][LGX_MAX_LEVELS + 1]: first step at or above each Hi-Z level
+  const T *dirs;   // [grid * grid_y][D][2]
+  const T *steps;  // [grid * grid_y][S_tab][2] = (offset, lod)
+  const int *level_start;  // [grid * grid_y][LGX_MAX_LEVELS + 1]: first step at or above each Hi-Z level
   const float *depth;
   int depth_pitch;
   const void *normal;
@@ -160,10 +160,10 @@ template <class T> LGX_D size_t point_index(T sx, T sy, int lw, int lh, int pitc
 
 // T: per-step arithmetic; A: accumulation of the slice contributions and of
 // the direction sum (float/float, double/double, or float/double).
-// One workgroup's share: logical block b (slice k = b % grid^2, tile b / grid^2).
+// One workgroup's share: logical block b (slice k = b % (grid grid_y), tile b / (grid grid_y)).
 template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT>
 LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
-  const int g2 = a.grid * a.grid;
+  const int g2 = a.grid * a.grid_y;
   const int k = (int)(b % (uint32_t)g2);
   const int t = (int)(b / (uint32_t)g2);
   const int kx = k % a.grid, ky = k / a.grid;
@@ -171,7 +171,7 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
   const int ly = (t / a.tiles_x) * a.tile_h + (int)(threadIdx.x / (uint32_t)a.tile_w);
   if (lx >= a.lcells_x || ly >= a.lcells_y) return;
   const int sx = a.sx0 + lx, sy = a.sy0 + ly;
-  const int x = sx * a.grid + kx, y = sy * a.grid + ky;
+  const int x = sx * a.grid + kx, y = sy * a.grid_y + ky;
   if (x < a.rx0 || x >= a.rx1 || y < a.ry0 || y >= a.ry1) return;
 
   const Vec3<T> cam{a.cam[0], a.cam[1], a.cam[2]};
@@ -457,15 +457,16 @@ template <class T> static int table_steps(const lgx_gather_params *p) {
 
 static bool gather_params_ok(const lgx_gather_params *p) {
   return p && p->width > 0 && p->height > 0 && p->dirs >= 1 && p->dirs <= LGX_MAX_DIRS &&
-         p->footprint_dirs <= LGX_MAX_DIRS && p->grid >= 1 &&
-         p->grid <= 8 && (p->sampler == LGX_SAMPLER_TRILINEAR || p->sampler == LGX_SAMPLER_HIZ_POINT) &&
+         p->footprint_dirs <= LGX_MAX_DIRS && p->grid >= 1 && p->grid <= 16 && p->grid_y <= 16 &&
+         p->grid * (p->grid_y > 0 ? p->grid_y : p->grid) <= 64 &&
+         (p->sampler == LGX_SAMPLER_TRILINEAR || p->sampler == LGX_SAMPLER_HIZ_POINT) &&
          (p->precision == LGX_PREC_F32 || p->precision == LGX_PREC_F64 || p->precision == LGX_PREC_F32_ACC64) &&
          (p->out_layout == LGX_OUT_INTERLEAVED || p->out_layout == LGX_OUT_SLICES);
 }
 
 static GatherTableShape table_shape(const lgx_gather_params *p) {
   GatherTableShape s;
-  s.g2 = p->grid * p->grid;
+  s.g2 = p->grid * (p->grid_y > 0 ? p->grid_y : p->grid);
   s.D = p->dirs;
   s.S_tab = p->precision == LGX_PREC_F64 ? table_steps<double>(p) : table_steps<float>(p);
   s.elem = p->precision == LGX_PREC_F64 ? sizeof(double) : sizeof(float);
@@ -627,8 +628,9 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
   a.w = p->width;
   a.h = p->height;
   a.grid = p->grid;
-  a.cells_x = (p->width + p->grid - 1) / p->grid;
-  a.cells_y = (p->height + p->grid - 1) / p->grid;
+  a.grid_y = p->grid_y > 0 ? p->grid_y : p->grid;
+  a.cells_x = (p->width + a.grid - 1) / a.grid;
+  a.cells_y = (p->height + a.grid_y - 1) / a.grid_y;
   a.D = p->dirs;
   a.S_tab = s.S_tab;
   a.dirs = (const T *)ws;
@@ -668,10 +670,10 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
     a.rx1 = p->roi[0] + p->roi[2];
     a.ry1 = p->roi[1] + p->roi[3];
   }
-  a.sx0 = a.rx0 / p->grid;
-  a.sy0 = a.ry0 / p->grid;
-  a.lcells_x = (a.rx1 + p->grid - 1) / p->grid - a.sx0;
-  a.lcells_y = (a.ry1 + p->grid - 1) / p->grid - a.sy0;
+  a.sx0 = a.rx0 / a.grid;
+  a.sy0 = a.ry0 / a.grid_y;
+  a.lcells_x = (a.rx1 + a.grid - 1) / a.grid - a.sx0;
+  a.lcells_y = (a.ry1 + a.grid_y - 1) / a.grid_y - a.sy0;
   a.tiles_x = (a.lcells_x + a.tile_w - 1) / a.tile_w;
   int tiles_y = (a.lcells_y + a.tile_h - 1) / a.tile_h;
   a.n_blocks = a.tiles_x * tiles_y * s.g2;
@@ -705,7 +707,9 @@ static int check_gather(const lgx_gather_params *p, const void *workspace, const
     return LGX_ESIZE;
   if (out->w != p->width || out->h != p->height) return LGX_ESIZE;
   if (out->format != (p->precision == LGX_PREC_F32 ? LGX_RGBA32F : LGX_RGBA64F)) return LGX_EFORMAT;
-  if (p->out_layout == LGX_OUT_SLICES && (p->width % p->grid || p->height % p->grid)) return LGX_ESIZE;
+  if (p->out_layout == LGX_OUT_SLICES &&
+      (p->width % p->grid || p->height % (p->grid_y > 0 ? p->grid_y : p->grid)))
+    return LGX_ESIZE;
   if (p->roi[2] > 0 && (p->roi[0] < 0 || p->roi[1] < 0 || p->roi[3] <= 0 || p->roi[0] + p->roi[2] > p->width ||
                         p->roi[1] + p->roi[3] > p->height))
     return LGX_ESIZE;

@@ -137,6 +137,13 @@ struct SsvgiRenderer::ViewportResources {
     // the workspace depends on the resolution and parameters only: fill it once
     Check(lgx_gather_prepare(&p, rg->BufferPointer(workspace.Id())), "lgx_gather_prepare");
   }
+  // The pattern divides the frame: the gather writes slices and the
+  // InterleavePass restores the image (SSVGIRenderer.h's structure); else
+  // the gather writes the image directly.
+  static bool Sliced(const SsvgiConfig &c) {
+    const int gy = c.grid_y > 0 ? c.grid_y : c.grid;
+    return c.width % c.grid == 0 && c.height % gy == 0;
+  }
   static lgx_gather_params GatherParams(const SsvgiConfig &c, const float *view, const float *proj) {
     lgx_gather_params p;
     memset(&p, 0, sizeof(p));
@@ -147,9 +154,10 @@ struct SsvgiRenderer::ViewportResources {
     p.dirs = c.dirs;
     p.max_steps = c.max_steps;
     p.grid = c.grid;
+    p.grid_y = c.grid_y;
     p.sampler = c.sampler;
     p.precision = LGX_PREC_F32;
-    p.out_layout = LGX_OUT_SLICES;
+    p.out_layout = Sliced(c) ? LGX_OUT_SLICES : LGX_OUT_INTERLEAVED;
     p.tile = c.tile;
     p.footprint_dirs = c.footprint_dirs;
     return p;
@@ -173,7 +181,9 @@ SsvgiRenderer::~SsvgiRenderer() = default;
 void SsvgiRenderer::RecreateSwapchainResources(uvec2 size) {
   cfg_.width = (int32_t)size.x;
   cfg_.height = (int32_t)size.y;
-  if (cfg_.levels < 1 || cfg_.levels > LGX_MAX_LEVELS || cfg_.grid < 1 || size.x % cfg_.grid || size.y % cfg_.grid)
+  const int gy = cfg_.grid_y > 0 ? cfg_.grid_y : cfg_.grid;
+  if (cfg_.levels < 1 || cfg_.levels > LGX_MAX_LEVELS || cfg_.grid < 1 || cfg_.grid > 16 || gy > 16 ||
+      cfg_.grid * gy > 64)
     throw Error(LGX_EUNSUPPORTED, "SsvgiRenderer: levels / grid");
   vr_.reset();
   vr_.reset(new ViewportResources(rg_, cfg_, size));
@@ -209,7 +219,7 @@ void SsvgiRenderer::RenderFrame(const SsvgiFrame &f) {
     std::vector<RenderGraph::ImageViewProxyId> inputs = dlev;
     inputs.insert(inputs.end(), rlev.begin(), rlev.end());
     inputs.push_back(v.normal.Id());
-    auto out = v.indirectSlices.imageViewProxy.Id();
+    auto out = ViewportResources::Sliced(c) ? v.indirectSlices.imageViewProxy.Id() : v.indirect.imageViewProxy.Id();
     auto ws = v.workspace.Id();
     auto normal = v.normal.Id();
     lgx_gather_params p = ViewportResources::GatherParams(c, f.view, f.proj);
@@ -235,8 +245,9 @@ void SsvgiRenderer::RenderFrame(const SsvgiFrame &f) {
                        Check(lgx_gather(&pp, ctx.GetBuffer(ws), &gb, &pyr, &o, ctx.GetStream()), "lgx_gather");
                      }));
   }
-  interleaveBuilder_.Interleave(rg_, v.indirectSlices.imageViewProxy.Id(), v.indirect.imageViewProxy.Id(),
-                                uvec2{(uint32_t)c.grid, (uint32_t)c.grid});
+  if (ViewportResources::Sliced(c))
+    interleaveBuilder_.Interleave(rg_, v.indirectSlices.imageViewProxy.Id(), v.indirect.imageViewProxy.Id(),
+                                  uvec2{(uint32_t)c.grid, (uint32_t)(c.grid_y > 0 ? c.grid_y : c.grid)});
   // :265-302 DenoiserPass
   RenderGraph::ImageViewProxyId cur = v.indirect.imageViewProxy.Id();
   if (c.denoiser == LGX_DENOISER_UNET) {
@@ -367,6 +378,7 @@ extern "C" int lgx_renderer_create(const lgx_renderer_config *c, lgx_renderer **
     s.footprint_dirs = c->footprint_dirs;
     s.levels = c->levels;
     s.grid = c->grid;
+    s.grid_y = c->grid_y;
     s.sampler = c->sampler;
     s.denoiser = c->denoiser;
     s.denoise_passes = c->denoise_passes;

@@ -3,8 +3,8 @@ training loop on the BF16 MFMA U-Net, and evaluation against the
 256-direction FP64 gather.
 
 Pairs: a seeded scene (32 boxes, 8 spheres) seen from a seeded camera pose
-at 1920x1080; a 256x256 crop (origin on the 8-pixel grid, so the interleave
-pattern phase matches full frames) of the 4-direction gather is the input,
+at 1920x1080; a 256x256 crop (origin on the 16-pixel grid, so the
+interleave pattern phase matches full frames) of the 4-direction gather is the input,
 the 256-direction gather accumulated in FP64 on the same crop the target.
 The input gather marches with the step growth and mip lod of a larger
 direction count (footprint_dirs) and without a step cap: the error of a
@@ -88,9 +88,10 @@ class FrameGen:
 def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx.PREC_F32_ACC64,
                   footprints=(NOISY_FOOTPRINT,), grid=NOISY_GRID):
     """Inputs X[f] [n, CROP, CROP, 12] for each footprint f and targets T
-    [n, CROP, CROP, 4], all on the GPU.  The input gather uses a grid x grid
-    interleave pattern; crop origins are multiples of 8 pixels, so the
-    pattern phase matches full frames for grid 4 and 8."""
+    [n, CROP, CROP, 4], all on the GPU.  The input gather uses the
+    interleave pattern `grid` (an int: square, or (width, height)); crop
+    origins are multiples of 16 pixels, so the pattern phase matches full
+    frames for pattern extents dividing 16."""
     fg = FrameGen()
     n = n_scenes * poses
     X = {f: torch.empty((n, CROP, CROP, 12), dtype=torch.float32, device="cuda") for f in footprints}
@@ -102,8 +103,8 @@ def build_dataset(n_scenes, poses=4, seed0=1000, log=print, target_precision=lgx
     for i in range(n):
         s, p = seed0 + i // poses, i % poses
         view, proj = fg.view(s, p)
-        x0 = int(rng.integers(0, (W - CROP) // 8 + 1)) * 8
-        y0 = int(rng.integers(0, (H - CROP) // 8 + 1)) * 8
+        x0 = int(rng.integers(0, (W - CROP) // 16 + 1)) * 16
+        y0 = int(rng.integers(0, (H - CROP) // 16 + 1)) * 16
         roi = (x0, y0, CROP, CROP)
         for f in footprints:
             fg.gather(NOISY_DIRS, NOISY_STEPS, footprint=f, grid=grid)(view, proj, fg.g, fg.pyr, out=fg.out32,
@@ -195,12 +196,12 @@ def main(argv=None):
     t.add_argument("--footprint", type=int, default=NOISY_FOOTPRINT)
     t.add_argument("--schedule", default="cosine", choices=("cosine", "const"))
     t.add_argument("--loss", default="l1", choices=("l1", "l2"))
-    t.add_argument("--grid", type=int, default=NOISY_GRID)
+    t.add_argument("--grid", type=int, nargs="+", default=None, help="pattern width [height]")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", default="200,201,202,203,204,205,206,207")
     e.add_argument("--footprint", type=int, default=NOISY_FOOTPRINT)
-    e.add_argument("--grid", type=int, default=NOISY_GRID)
+    e.add_argument("--grid", type=int, nargs="+", default=None, help="pattern width [height]")
     args = ap.parse_args(argv)
     dev.require_device()
 
@@ -208,6 +209,7 @@ def main(argv=None):
         print(msg, flush=True)
 
     if args.cmd == "train":
+        args.grid = NOISY_GRID if args.grid is None else (args.grid[0] if len(args.grid) == 1 else tuple(args.grid))
         X, T = build_dataset(args.scenes, args.poses, log=log, footprints=(args.footprint,), grid=args.grid)
         X = X[args.footprint]
         net = dev.UNet(args.batch, CROP, CROP, seed=0)
@@ -232,6 +234,7 @@ def main(argv=None):
                            "schedule": args.schedule, "loss": args.loss, "lr": args.lr, "grid": args.grid,
                            "resume": args.resume, "eval": rows}, f)
     else:
+        args.grid = NOISY_GRID if args.grid is None else (args.grid[0] if len(args.grid) == 1 else tuple(args.grid))
         net = dev.UNet(1, H, W)
         net.load(args.weights)
         evaluate(net, [int(s) for s in args.seeds.split(",") if s], footprint=args.footprint, grid=args.grid)

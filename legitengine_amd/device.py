@@ -135,7 +135,8 @@ class Gather:
     def __init__(self, w, h, dirs, max_steps, sampler, grid=4, precision=lgx.PREC_F32, out_layout=lgx.OUT_INTERLEAVED,
                  tile=0, footprint_dirs=0):
         p = lgx.GatherParams()
-        p.width, p.height, p.dirs, p.max_steps, p.grid = w, h, dirs, max_steps, grid
+        p.width, p.height, p.dirs, p.max_steps = w, h, dirs, max_steps
+        p.grid, p.grid_y = lgx.pattern(grid)
         p.footprint_dirs = footprint_dirs
         p.sampler, p.precision, p.out_layout, p.tile = sampler, precision, out_layout, tile
         self.p = p
@@ -207,7 +208,8 @@ class Renderer:
                  composite=False, profile=False, validate=False, footprint_dirs=0):
         f = lgx.GBUF_FORMATS[kind]
         c = lgx.RendererConfig()
-        c.width, c.height, c.dirs, c.max_steps, c.levels, c.grid, c.sampler = w, h, dirs, max_steps, levels, grid, sampler
+        c.width, c.height, c.dirs, c.max_steps, c.levels, c.sampler = w, h, dirs, max_steps, levels, sampler
+        c.grid, c.grid_y = lgx.pattern(grid)
         c.denoiser, c.denoise_passes, c.denoise_radius = denoiser, denoise_passes, denoise_radius
         c.depth_sharpness, c.normal_sharpness = 10.5, 20.0
         c.normal_format, c.albedo_format, c.radiance_format = f["normal"], f["albedo"], f["radiance"]
@@ -436,7 +438,8 @@ class Batch:
             vs[i].view[:] = [float(v) for v in view]
             vs[i].proj[:] = [float(v) for v in proj]
         c = lgx.BatchConfig()
-        c.dirs, c.max_steps, c.levels, c.grid, c.sampler = dirs, max_steps, levels, grid, sampler
+        c.dirs, c.max_steps, c.levels, c.sampler = dirs, max_steps, levels, sampler
+        c.grid, c.grid_y = lgx.pattern(grid)
         c.footprint_dirs, c.tile, c.workgroups = footprint_dirs, tile, workgroups
         self._keep = (views, net)
         self.n = n

@@ -23,6 +23,11 @@ UNET_LOSS_L1, UNET_LOSS_L2 = 0, 1
 OK, EINVAL, EFORMAT, ESIZE, EUNSUPPORTED, EWORKSPACE, ENODEVICE = 0, -1, -2, -3, -4, -5, -6
 
 
+def pattern(grid):
+    """Interleave pattern (width, height) from an int (square) or a pair."""
+    return (int(grid), int(grid)) if isinstance(grid, int) else (int(grid[0]), int(grid[1]))
+
+
 class Image(C.Structure):
     _fields_ = [("ptr", C.c_void_p), ("w", C.c_int32), ("h", C.c_int32), ("pitch", C.c_int32),
                 ("format", C.c_int32), ("reserved", C.c_int32)]
@@ -40,7 +45,7 @@ class GatherParams(C.Structure):
     _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("width", C.c_int32), ("height", C.c_int32),
                 ("dirs", C.c_int32), ("max_steps", C.c_int32), ("grid", C.c_int32), ("sampler", C.c_int32),
                 ("precision", C.c_int32), ("out_layout", C.c_int32), ("step_counts", C.c_void_p),
-                ("tile", C.c_int32), ("roi", C.c_int32 * 4), ("footprint_dirs", C.c_int32)]
+                ("tile", C.c_int32), ("roi", C.c_int32 * 4), ("footprint_dirs", C.c_int32), ("grid_y", C.c_int32)]
 
 
 class BilateralParams(C.Structure):
@@ -63,7 +68,8 @@ class RendererConfig(C.Structure):
                 ("denoise_passes", C.c_int32), ("denoise_radius", C.c_int32), ("depth_sharpness", C.c_float),
                 ("normal_sharpness", C.c_float), ("normal_format", C.c_int32), ("albedo_format", C.c_int32),
                 ("radiance_format", C.c_int32), ("tile", C.c_int32), ("composite", C.c_int32),
-                ("profile", C.c_int32), ("validate", C.c_int32), ("footprint_dirs", C.c_int32)]
+                ("profile", C.c_int32), ("validate", C.c_int32), ("footprint_dirs", C.c_int32),
+                ("grid_y", C.c_int32)]
 
 
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_REGRESSION, DENOISER_UNET = 0, 1, 2, 3
@@ -77,7 +83,7 @@ class BatchView(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [("dirs", C.c_int32), ("max_steps", C.c_int32), ("levels", C.c_int32), ("grid", C.c_int32),
                 ("sampler", C.c_int32), ("footprint_dirs", C.c_int32), ("tile", C.c_int32),
-                ("workgroups", C.c_int32)]
+                ("workgroups", C.c_int32), ("grid_y", C.c_int32)]
 
 # Every entry point declared in include/lgx.h, with its ctypes signature.
 SIGNATURES = {

@@ -20,7 +20,7 @@ _lib = None
 class GatherParams(C.Structure):
     _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("width", C.c_int32), ("height", C.c_int32),
                 ("dirs", C.c_int32), ("max_steps", C.c_int32), ("grid", C.c_int32), ("sampler", C.c_int32),
-                ("n_levels", C.c_int32), ("footprint_dirs", C.c_int32)]
+                ("n_levels", C.c_int32), ("footprint_dirs", C.c_int32), ("grid_y", C.c_int32)]
 
 
 FP = C.POINTER(C.c_float)
@@ -149,13 +149,14 @@ def build_pyramids(depth, radiance, n_levels, sampler, radiance_fp16):
 
 
 def gather(view, proj, depth, oct_normal, pyr_depth, pyr_rad, dirs, max_steps, sampler, grid=4,
-           precision="f32", crop=None, threads=0, with_steps=False, footprint_dirs=0):
+           precision="f32", crop=None, threads=0, with_steps=False, footprint_dirs=0, grid_y=0):
     h, w = depth.shape
     p = GatherParams()
     p.footprint_dirs = footprint_dirs
     p.view[:] = [float(v) for v in view]
     p.proj[:] = [float(v) for v in proj]
-    p.width, p.height, p.dirs, p.max_steps, p.grid, p.sampler = w, h, dirs, max_steps, grid, sampler
+    gx, gy = (grid, grid_y) if isinstance(grid, int) else grid
+    p.width, p.height, p.dirs, p.max_steps, p.grid, p.grid_y, p.sampler = w, h, dirs, max_steps, gx, gy, sampler
     p.n_levels = len(pyr_depth)
     pd = (FP * len(pyr_depth))(*[fp(a) for a in pyr_depth])
     pr = (FP * len(pyr_rad))(*[fp(a) for a in pyr_rad])

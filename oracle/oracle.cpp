@@ -299,11 +299,11 @@ void gather_pixel(const orc_gather_params &p, const M4<T> &invVP, V3<T> cam, int
   V3<T> rayc = normalize(unproject(V3<T>{s.x, s.y, T(1)}, invVP) - cam);
   V3<T> P = cam + rayc * rc; /* :130 */
 
-  const int g = p.grid;
-  int pix = px % g, piy = py % g; /* :155 */
-  int index = pix + piy * g;      /* :161 */
+  const int g = p.grid, gy = p.grid_y > 0 ? p.grid_y : p.grid; /* square in the shader */
+  int pix = px % g, piy = py % gy; /* :155 */
+  int index = pix + piy * g;       /* :161 */
   float angf, linf;
-  hammersley(index, g * g, angf, linf); /* :162-164 */
+  hammersley(index, g * gy, angf, linf); /* :162-164 */
   const T ang_offset = T(angf), lin_offset = T(linf);
   const T pi = T(3.1415f);             /* :169, verbatim */
   const T near_step = W / T(1000);     /* :167 */

@@ -34,6 +34,7 @@ typedef struct {
   int32_t sampler;   /* 0 = trilinear over avg+blur pyramids (reference), 1 = Hi-Z point fetch at floor(lod) */
   int32_t n_levels;  /* pyramid levels present */
   int32_t footprint_dirs; /* direction count in the step growth (:212, :217) and lod (:234); <= 0 = dirs */
+  int32_t grid_y;    /* pattern height, <= 0 = grid: index (x % grid) + (y % grid_y) * grid */
 } orc_gather_params;
 
 uint32_t orc_version(void);

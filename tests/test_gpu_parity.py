@@ -133,10 +133,11 @@ def test_gather_c1(dev, c1, seed_idx, sampler, dirs, steps):
     assert np.array_equal(counts, ref_steps)
 
 
-@pytest.mark.parametrize("footprint,steps,grid", [(32, 64, 4), (64, 64, 4), (256, 0, 4), (256, 0, 8)])
+@pytest.mark.parametrize("footprint,steps,grid", [(32, 64, 4), (64, 64, 4), (256, 0, 4), (256, 0, 8),
+                                                  (256, 0, (4, 16))])
 def test_gather_footprint(dev, c1, footprint, steps, grid):
     """4 directions marching with a larger direction count's footprint, with
-    or without a step cap, on a 4x4 or 8x8 interleave pattern (the
+    or without a step cap, on a 4x4, 8x8 or 4x16 interleave pattern (the
     denoiser's input), against the oracle with the same parameters."""
     sc, view, proj, gb = c1[1]
     g = dev.GBufferDev.from_host(gb)

@@ -108,18 +108,23 @@ def test_renderer_rejects_bad_config(dev):
     c.width, c.height, c.dirs, c.max_steps, c.levels, c.grid, c.sampler = 66, 64, 16, 32, 5, 4, 1
     c.normal_format, c.albedo_format, c.radiance_format = lgx.RG16F, lgx.RGBA8, lgx.RGBA16F
     h = C.c_void_p()
-    assert lgx.lib().lgx_renderer_create(C.byref(c), C.byref(h)) == lgx.EUNSUPPORTED  # 4 does not divide 66
-    c.width, c.dirs = 64, 0
+    c.grid, c.grid_y = 8, 16  # 128 pattern offsets: more than the gather's tables hold
+    assert lgx.lib().lgx_renderer_create(C.byref(c), C.byref(h)) == lgx.EUNSUPPORTED
+    c.grid, c.grid_y, c.width, c.dirs = 4, 0, 64, 0
     assert lgx.lib().lgx_renderer_create(C.byref(c), C.byref(h)) == lgx.EUNSUPPORTED
     assert not h.value
 
 
-@pytest.mark.parametrize("footprint,steps,grid", [(0, 64, 4), (32, 64, 4), (256, 0, 8)])
-def test_unet_denoiser_pass(dev, footprint, steps, grid):
+@pytest.mark.parametrize("footprint,steps,grid,h", [(0, 64, 4, 192), (32, 64, 4, 192), (256, 0, 8, 192),
+                                                    (256, 0, (4, 16), 192), (256, 0, (4, 16), 200)])
+def test_unet_denoiser_pass(dev, footprint, steps, grid, h):
     """LGX_DENOISER_UNET: the renderer's DenoiserPass equals lgx_unet_denoise
-    run on the renderer's own deinterleaved gather output, bit for bit, and
-    the gather honours footprint_dirs, the step cap and the pattern size."""
-    w, h = 320, 192
+    run on the renderer's own gather output, bit for bit, and the gather
+    honours footprint_dirs, the step cap and the pattern size; the image
+    equals the gather's own interleaved output, through slices and the
+    InterleavePass when the pattern divides the frame (16 does not divide
+    200: the gather writes the image directly)."""
+    w = 320
     view, proj, gb, g = frame_inputs(dev, 23, w, h, "mixed")
     net = dev.UNet(1, h, w, seed=5)
     r = dev.Renderer(w, h, 4, steps, 6, lgx.SAMPLER_HIZ_POINT, grid, kind="mixed", denoiser=lgx.DENOISER_UNET,
@@ -131,9 +136,14 @@ def test_unet_denoiser_pass(dev, footprint, steps, grid):
     torch.cuda.synchronize()
     ind = r.intermediate("indirect")
     pyr = dev.PyramidDev(g, 6, lgx.SAMPLER_HIZ_POINT).build()
-    sl = dev.Gather(w, h, 4, steps, lgx.SAMPLER_HIZ_POINT, grid=grid, out_layout=lgx.OUT_SLICES,
-                    footprint_dirs=footprint)(view, proj, g, pyr)
-    assert dev.permute(sl, inverse=True, gx=grid, gy=grid).numpy().tobytes() == ind.tobytes()
+    direct = dev.Gather(w, h, 4, steps, lgx.SAMPLER_HIZ_POINT, grid=grid, footprint_dirs=footprint)(view, proj, g,
+                                                                                                pyr)
+    assert direct.numpy().tobytes() == ind.tobytes()
+    gx, gy = lgx.pattern(grid)
+    if w % gx == 0 and h % gy == 0:
+        sl = dev.Gather(w, h, 4, steps, lgx.SAMPLER_HIZ_POINT, grid=grid, out_layout=lgx.OUT_SLICES,
+                        footprint_dirs=footprint)(view, proj, g, pyr)
+        assert dev.permute(sl, inverse=True, gx=gx, gy=gy).numpy().tobytes() == ind.tobytes()
     noisy = dev.DImage.from_numpy(ind, lgx.RGBA32F)
     ref = net.denoise(g, noisy).numpy()
     assert r.output().tobytes() == ref.tobytes()

@@ -386,10 +386,10 @@ def test_gather_plane_grid8():
     assert abs(o0[24, 32, 0] / 0.01 - 1.0) < 3e-2
 
 
-@pytest.mark.parametrize("grid", [4, 8])
+@pytest.mark.parametrize("grid", [4, 8, (4, 16)])
 def test_pattern_index_and_direction(grid):
-    """Pins the pattern index (x % g) + (y % g) g and the per-pattern screen
-    direction 2 pi (k / g^2) / D (pi = 3.1415f, :169) through the executed
+    """Pins the pattern index (x % gx) + (y % gy) gx and the per-pattern
+    screen direction 2 pi (k / (gx gy)) / D (pi = 3.1415f, :169) through the executed
     step counts of a one-direction gather, recomputed independently in
     float64: the count is the number of offsets off_j = near ratio^(j + lin)
     + 1 - near (:212-217) whose sample stays strictly inside the screen.  A
@@ -399,11 +399,12 @@ def test_pattern_index_and_direction(grid):
     _, steps = orc.gather(view, P, depth, octn, [depth], [rad], 1, 0, 0, grid=grid, precision="f64",
                           with_steps=True, footprint_dirs=16)
     pi = float(np.float32(3.1415))
-    g2 = grid * grid
+    gx, gy = (grid, grid) if isinstance(grid, int) else grid
+    g2 = gx * gy
     near, ratio = w / 1000.0, 2 * pi / 16 + 1
 
     def count(x, y, kx, ky):
-        k = kx + ky * grid
+        k = kx + ky * gx
         ang = 2 * pi / 1 * (k / g2)
         lin = int(f"{k:0{g2.bit_length() - 1}b}"[::-1], 2) / g2
         dx, dy = np.cos(ang), np.sin(ang)
@@ -420,11 +421,12 @@ def test_pattern_index_and_direction(grid):
         return n
 
     ys, xs = np.mgrid[0:h, 0:w]
-    ref = np.array([[count(x, y, x % grid, y % grid) for x in range(w)] for y in range(h)])
+    ref = np.array([[count(x, y, x % gx, y % gy) for x in range(w)] for y in range(h)])
     assert (ref == steps).mean() > 0.995, (ref == steps).mean()
-    if grid > 1:  # the transposed index is far off
-        bad = np.array([[count(x, y, y % grid, x % grid) for x in range(w)] for y in range(h)])
-        assert (bad == steps).mean() < 0.6
+    # the transposed index (square patterns) or the square index (4 x 16) is far off
+    bad = np.array([[count(x, y, (y % gx) if gx == gy else (x % gx), (x % gy) if gx == gy else (y % gx))
+                     for x in range(w)] for y in range(h)])
+    assert (bad == steps).mean() < 0.6
 
 
 def test_gather_linear_in_radiance_and_float_vs_double():

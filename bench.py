@@ -319,11 +319,11 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     eq_ms = min(fam.values()) if fam else None
     for e in plain.values():
         e["renderer"].close()
+    gx, gy = lgx.pattern(dn.NOISY_GRID)
     return {
         "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather (step growth and lod of {dn.NOISY_FOOTPRINT} "
-                    f"directions, reference step rule without cap), {LEVELS}-level Hi-Z, "
-                    f"{dn.NOISY_GRID}x{dn.NOISY_GRID} interleave, BF16 U-Net "
-                    "denoiser pass (3 levels, 32/64/128 channels, FP32 accumulation)",
+                    f"directions, reference step rule without cap), {LEVELS}-level Hi-Z, {gx}x{gy} interleave, "
+                    "BF16 U-Net denoiser pass (3 levels, 32/64/128 channels, FP32 accumulation)",
         **timing,
         "rel_l2_vs_fp64_256": round(err, 5),
         "rel_l2_per_seed": [round(x, 5) for x in den_err],

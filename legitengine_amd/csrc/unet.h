@@ -43,7 +43,9 @@ struct WgradArgs {
   int n, h, w;
   int rows;        // rows per workgroup (set by conv3x3_wgrad)
   float *dw;       // [9][ca+cb][cout], overwritten
+  float *db;       // [cout] bias gradient, overwritten
   float *part;     // scratch for the per-workgroup partial sums
+  float *part_b;   // (set by conv3x3_wgrad) bias partials inside part
   size_t part_bytes;
 };
 
@@ -52,7 +54,6 @@ int conv3x3_tiles(int cout);  // register tiles offered for this output channel 
 int conv3x3_tile_lds(int cout, int tile);  // 1 if that tile is an LDS tile covering all cout channels
 hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s);
 size_t conv3x3_wgrad_scratch_bytes();
-hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s);
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);
 hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12, hipStream_t s);
 // inference from the G-buffer: the BF16 input records and the head with the noisy image as residual

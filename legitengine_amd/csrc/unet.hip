@@ -221,10 +221,10 @@ class Net {
     g.h = h;
     g.w = w;
     g.dw = dw[l];
+    g.db = grads + b_off[l];
     g.part = wpart;
     g.part_bytes = wpart_bytes;
     check(conv3x3_wgrad(g, s), "conv3x3_wgrad");
-    check(channel_sum(dy, kLayers[l].cout, (size_t)n * h * w, grads + b_off[l], s), "channel_sum");
   }
 
   // forward convolution l on the activations of an n x h x w input

@@ -419,7 +419,9 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
 // Tiles (tap, co block, ci block) are spread over the 4 waves; each
 // workgroup stores its partial sums (no atomics) to its own slot of the
 // scratch buffer, part[group][slot][tap][ci % 32][co], and
-// wgrad_reduce_kernel adds the slots in a fixed order.
+// wgrad_reduce_kernel adds the slots in a fixed order.  The workgroups of
+// input group 0 also sum the staged output-gradient rows per channel (the
+// bias gradient) into part_b[slot][co], reduced by bias_reduce_kernel.
 template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const WgradArgs a) {
   constexpr int XS = 64 + 16;  // staged x range [x0 - 8, x0 + 72): keeps dx = 0 reads 16-byte aligned
   constexpr int RB = 2;        // output rows per staging round
@@ -438,6 +440,11 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
   f32x4 acc[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // bias partial: thread t sums channel t % COUT over its eighth-, quarter-
+  // or half-row slice of pixels (BP = 256 / COUT slices)
+  constexpr int BP = 256 / COUT, BW = 64 / BP;
+  const int bc = threadIdx.x % COUT, bq = threadIdx.x / COUT;
+  float bsum = 0.f;
   typedef unsigned short u16;
   auto pair = [](bf16 lo, bf16 hi) {
     return (uint32_t)__builtin_bit_cast(u16, lo) | ((uint32_t)__builtin_bit_cast(u16, hi) << 16);
@@ -472,6 +479,15 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
       for (int i = 0; i < 8; i++) *(uint32_t *)&ds[rr][q * 8 + i][pp * 2] = pair(v0[i], v1[i]);
     }
     __syncthreads();
+    if (g == 0) {  // uniform across the workgroup
+      for (int rr = 0; rr < nr; rr++)
+#pragma unroll
+        for (int k = 0; k < BW; k += 8) {
+          const bf16x8 v = *(const bf16x8 *)&ds[rr][bc][bq * BW + k];
+#pragma unroll
+          for (int i = 0; i < 8; i++) bsum += bf2f(v[i]);
+        }
+    }
 #pragma unroll
     for (int ti = 0; ti < TPW; ti++) {
       const int t = wave + ti * 4;
@@ -516,6 +532,35 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
       *(f32x4 *)&dst[(tap * 32 + cb * 16 + col) * COUT + mb * 16 + kg * 4] = acc[ti];
     }
   }
+  if (g == 0) {
+    // the BP slices of each channel, added in a fixed order (xs is free now)
+    float *red = (float *)&xs[0][0][0];
+    red[bq * COUT + bc] = bsum;
+    __syncthreads();
+    if (threadIdx.x < COUT) {
+      float b = 0.f;
+#pragma unroll
+      for (int k = 0; k < BP; k++) b += red[k * COUT + threadIdx.x];
+      a.part_b[slot * COUT + threadIdx.x] = b;
+    }
+  }
+}
+
+// db[co] = sum over slots of part_b[slot][co], one workgroup per channel,
+// 256 strided partial sums then a fixed tree.
+__global__ __launch_bounds__(256) void bias_reduce_kernel(const float *part_b, int slots, int cout, float *db) {
+  __shared__ float red[256];
+  const int co = blockIdx.x;
+  float s = 0.f;
+  for (int k = threadIdx.x; k < slots; k += 256) s += part_b[(size_t)k * cout + co];
+  red[threadIdx.x] = s;
+  __syncthreads();
+#pragma unroll
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[co] = red[0];
 }
 
 // dw[tap][g * 32 + cl][co] = sum over slots of part[g][slot][tap][cl][co].
@@ -549,31 +594,6 @@ __global__ __launch_bounds__(256) void wgrad_reduce_kernel(const f32x4 *part, in
   }
 }
 
-// Per-channel sums of an NHWC tensor (bias gradients), atomically added.
-__global__ __launch_bounds__(256) void channel_sum_kernel(const bf16 *x, int c, size_t npix, float *out) {
-  // thread: one 8-channel piece (gi) of pixels pi, pi + per, ... (16-byte
-  // loads); the block then adds the threads of each piece in LDS and issues
-  // one atomic per channel
-  __shared__ float red[256][9];
-  const int groups = c >> 3, per = 256 / groups;
-  const int gi = threadIdx.x % groups, pi = threadIdx.x / groups;
-  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (pi < per)
-    for (size_t p = (size_t)blockIdx.x * per + pi; p < npix; p += (size_t)gridDim.x * per) {
-      const bf16x8 v = *(const bf16x8 *)(x + p * c + gi * 8);
-#pragma unroll
-      for (int i = 0; i < 8; i++) acc[i] += bf2f(v[i]);
-    }
-#pragma unroll
-  for (int i = 0; i < 8; i++) red[threadIdx.x][i] = acc[i];
-  __syncthreads();
-  if ((int)threadIdx.x < c) {
-    const int g = threadIdx.x >> 3, i = threadIdx.x & 7;
-    float sum = 0.f;
-    for (int q = 0; q < per; q++) sum += red[q * groups + g][i];
-    atomicAdd(&out[threadIdx.x], sum);
-  }
-}
 
 // ------------------------------------------------------------ glue kernels
 
@@ -1026,13 +1046,14 @@ static int wgrad_target_blocks(int cout) { return cout <= 32 ? 1024 : cout <= 64
 
 size_t conv3x3_wgrad_scratch_bytes() {
   size_t m = 0;
-  for (int cout : {32, 64, 128}) m = std::max(m, (size_t)2 * wgrad_target_blocks(cout) * 9 * 32 * cout * 4);
+  for (int cout : {32, 64, 128}) m = std::max(m, (size_t)2 * wgrad_target_blocks(cout) * (9 * 32 + 1) * cout * 4);
   return m;
 }
 
 hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
   if (a.cout != 32 && a.cout != 64 && a.cout != 128) return hipErrorInvalidValue;
-  if ((a.ca + a.cb) % 32 || a.ca % 32 || a.n < 1 || a.h < 1 || a.w < 1 || !a.part) return hipErrorInvalidValue;
+  if ((a.ca + a.cb) % 32 || a.ca % 32 || a.n < 1 || a.h < 1 || a.w < 1 || !a.part || !a.dw || !a.db)
+    return hipErrorInvalidValue;
   const int groups = (a.ca + a.cb) / 32, bands = (a.w + 63) / 64, target = wgrad_target_blocks(a.cout);
   const long segs = (long)bands * a.n * groups;  // workgroups per row of rows
   // rows per workgroup: even, about `target` workgroups, at most 2 * target
@@ -1044,8 +1065,10 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
     blocks = segs * ((a.h + rows - 1) / rows);
   }
   const size_t per = (size_t)9 * 32 * a.cout;
-  if ((size_t)blocks * per * 4 > a.part_bytes) return hipErrorInvalidValue;
+  const int slots = (int)(blocks / groups);
+  if (((size_t)blocks * per + (size_t)slots * a.cout) * 4 > a.part_bytes) return hipErrorInvalidValue;
   a.rows = rows;
+  a.part_b = a.part + (size_t)blocks * per;
   dim3 grid(bands, (a.h + rows - 1) / rows, a.n * groups), block(256);
   switch (a.cout / 16) {
     case 2: hipLaunchKernelGGL(conv3x3_wgrad_kernel<2>, grid, block, 0, s, a); break;
@@ -1054,7 +1077,7 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int slots = (int)(blocks / groups), per4 = (int)(per / 4);
+  const int per4 = (int)(per / 4);
   const size_t total = (size_t)groups * per4;
   const f32x4 *part = (const f32x4 *)a.part;
   const int cin = a.ca + a.cb;
@@ -1067,16 +1090,10 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
   else if (slots <= 256) launch(wgrad_reduce_kernel<16>, 16);
   else if (slots <= 512) launch(wgrad_reduce_kernel<32>, 32);
   else launch(wgrad_reduce_kernel<64>, 64);
+  hipLaunchKernelGGL(bias_reduce_kernel, dim3(a.cout), dim3(256), 0, s, (const float *)a.part_b, slots, a.cout, a.db);
   return hipGetLastError();
 }
 
-hipError_t channel_sum(const bf16 *x, int c, size_t npix, float *out, hipStream_t s) {
-  if (c % 8 || c > 256 || 256 % (c / 8)) return hipErrorInvalidValue;
-  const size_t per = 256 / (c / 8);
-  hipLaunchKernelGGL(channel_sum_kernel, dim3((unsigned)std::min<size_t>(1024, (npix + per - 1) / per)), dim3(256), 0,
-                     s, x, c, npix, out);
-  return hipGetLastError();
-}
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s) {
   hipLaunchKernelGGL(pack_input_kernel, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, s, x12, npix, x32);
   return hipGetLastError();

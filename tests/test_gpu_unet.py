@@ -154,10 +154,10 @@ def test_denoise_from_gbuffer_matches_records(dev):
 def test_register_tiles_bit_identical_and_autotune(dev):
     """Every convolution register tile accumulates each output in the same
     order, so forward outputs are bit-identical whatever the tiles, and so
-    are the activation gradients and the convolution weight gradients (the
-    per-workgroup partial sums are reduced in a fixed order); bias and head
-    gradients differ only by the order of their FP32 atomic sums.  The
-    autotuner only picks among them."""
+    are the activation gradients and the convolution weight and bias
+    gradients (the per-workgroup partial sums are reduced in a fixed order);
+    the 1x1 head's gradients differ only by the order of their FP32 atomic
+    sums.  The autotuner only picks among them."""
     from legitengine_amd import lgx
     flat, x12, target = case(5, n=2, h=48, w=80)
     net = dev.UNet(2, 48, 80)
@@ -171,7 +171,7 @@ def test_register_tiles_bit_identical_and_autotune(dev):
     wmask = np.zeros(ref_grad.size, bool)
     off = 0
     for ci, co in U.LAYERS:
-        wmask[off:off + co * ci * 9] = True
+        wmask[off:off + co * ci * 9 + co] = True
         off += co * ci * 9 + co
     net.backward(x, t)
     assert net.get(1)[0][wmask].tobytes() == ref_grad[wmask].tobytes()  # same call twice

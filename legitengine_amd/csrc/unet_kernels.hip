@@ -416,7 +416,9 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
 // (channel-major, written as pixel pairs) so that MFMA operands are 8
 // consecutive pixels of one channel; K runs over pixels.  The horizontally
 // shifted input operands are two aligned 16-byte reads and an element shift.
-// Tiles (tap, co block, ci block) are spread over the 4 waves; each
+// Wave w owns input half cb = w & 1 and the output blocks mb = (w >> 1) + 2j,
+// all 9 taps: per (row, k chunk) it reads each of its A fragments once for
+// the 9 taps and each tap's B fragment once for its MB / 2 blocks.  Each
 // workgroup stores its partial sums (no atomics) to its own slot of the
 // scratch buffer, part[group][slot][tap][ci % 32][co], and
 // wgrad_reduce_kernel adds the slots in a fixed order.  The workgroups of
@@ -434,12 +436,13 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
   const int n = blockIdx.z / groups, g = blockIdx.z % groups;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * a.rows;
   const ConvInput in{a.a, a.ca, a.a_up, a.b, a.cb};
-  // tiles owned by this wave: t = wave, wave+4, ... over (tap, mb, cb) with cb in {0,1}
-  constexpr int NT = 9 * MB * 2;
-  constexpr int TPW = (NT + 3) / 4;
-  f32x4 acc[TPW];
+  constexpr int MJ = MB / 2;  // output blocks per wave
+  const int cb = wave & 1, mb0 = wave >> 1;
+  f32x4 acc[9][MJ];
 #pragma unroll
-  for (int i = 0; i < TPW; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 9; t++)
+#pragma unroll
+    for (int j = 0; j < MJ; j++) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   // bias partial: thread t sums channel t % COUT over its eighth-, quarter-
   // or half-row slice of pixels (BP = 256 / COUT slices)
   constexpr int BP = 256 / COUT, BW = 64 / BP;
@@ -488,33 +491,30 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
           for (int i = 0; i < 8; i++) bsum += bf2f(v[i]);
         }
     }
+    for (int rr = 0; rr < nr; rr++) {
 #pragma unroll
-    for (int ti = 0; ti < TPW; ti++) {
-      const int t = wave + ti * 4;
-      if (t < NT) {
-        const int tap = t / (MB * 2), rem = t % (MB * 2), mb = rem >> 1, cb = rem & 1;
-        const int dy = tap / 3, dx = tap % 3 - 1;
-        for (int rr = 0; rr < nr; rr++) {
+      for (int kc = 0; kc < 2; kc++) {
+        const int p0 = kc * 32 + kg * 8;
+        bf16x8 av[MJ];  // A: co x pixels
 #pragma unroll
-          for (int kc = 0; kc < 2; kc++) {
-            const int p0 = kc * 32 + kg * 8;
-            const bf16x8 av = *(const bf16x8 *)&ds[rr][mb * 16 + col][p0];  // A: co x pixels
-            const bf16 *base = &xs[rr + dy][cb * 16 + col][p0 + 8];         // B: pixels x ci, 16-byte aligned
-            bf16x8 bv;
-            if (dx == 0) {
-              bv = *(const bf16x8 *)base;
-            } else if (dx < 0) {  // elements base[-1 .. 6]
-              const bf16x8 lo = *(const bf16x8 *)(base - 8), hi = *(const bf16x8 *)base;
-              bv[0] = lo[7];
+        for (int j = 0; j < MJ; j++) av[j] = *(const bf16x8 *)&ds[rr][(mb0 + 2 * j) * 16 + col][p0];
 #pragma unroll
-              for (int i = 1; i < 8; i++) bv[i] = hi[i - 1];
-            } else {  // elements base[1 .. 8]
-              const bf16x8 lo = *(const bf16x8 *)base, hi = *(const bf16x8 *)(base + 8);
+        for (int dy = 0; dy < 3; dy++) {
+          const bf16 *base = &xs[rr + dy][cb * 16 + col][p0 + 8];  // B: pixels x ci, 16-byte aligned
+          const bf16x8 lo = *(const bf16x8 *)(base - 8), mid = *(const bf16x8 *)base,
+                       hi = *(const bf16x8 *)(base + 8);
+          bf16x8 bm, bp;  // dx = -1: elements base[-1 .. 6]; dx = +1: base[1 .. 8]
+          bm[0] = lo[7];
 #pragma unroll
-              for (int i = 0; i < 7; i++) bv[i] = lo[i + 1];
-              bv[7] = hi[0];
-            }
-            acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[ti], 0, 0, 0);
+          for (int i = 1; i < 8; i++) bm[i] = mid[i - 1];
+#pragma unroll
+          for (int i = 0; i < 7; i++) bp[i] = mid[i + 1];
+          bp[7] = hi[0];
+#pragma unroll
+          for (int j = 0; j < MJ; j++) {
+            acc[dy * 3 + 0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[j], bm, acc[dy * 3 + 0][j], 0, 0, 0);
+            acc[dy * 3 + 1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[j], mid, acc[dy * 3 + 1][j], 0, 0, 0);
+            acc[dy * 3 + 2][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[j], bp, acc[dy * 3 + 2][j], 0, 0, 0);
           }
         }
       }
@@ -525,13 +525,10 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
   const size_t slot = ((size_t)n * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   float *dst = a.part + ((size_t)g * slots + slot) * (9 * 32 * COUT);
 #pragma unroll
-  for (int ti = 0; ti < TPW; ti++) {
-    const int t = wave + ti * 4;
-    if (t < NT) {
-      const int tap = t / (MB * 2), rem = t % (MB * 2), mb = rem >> 1, cb = rem & 1;
-      *(f32x4 *)&dst[(tap * 32 + cb * 16 + col) * COUT + mb * 16 + kg * 4] = acc[ti];
-    }
-  }
+  for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+    for (int j = 0; j < MJ; j++)
+      *(f32x4 *)&dst[(tap * 32 + cb * 16 + col) * COUT + (mb0 + 2 * j) * 16 + kg * 4] = acc[tap][j];
   if (g == 0) {
     // the BP slices of each channel, added in a fixed order (xs is free now)
     float *red = (float *)&xs[0][0][0];

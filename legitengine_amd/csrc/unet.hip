@@ -373,10 +373,10 @@ class Net {
     const size_t P = (size_t)n * h * w;
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
     forward(x12, n, h, w, out, s);
-    check(fill(grads, np, 0.f, s), "fill");
+    check(fill(grads, np, 0.f, s), "fill");  // the head's gradient is accumulated atomically
     check(fill(loss, 1, 0.f, s), "fill");
-    check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P), loss_kind == LGX_UNET_LOSS_L2, g_d1,
-                   grads + head_w_off, grads + head_b_off, loss, s),
+    check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P),
+                   loss_kind == LGX_UNET_LOSS_L2, g_d1, grads + head_w_off, grads + head_b_off, loss, s),
           "head_bwd");
     // d1 = relu(conv9(d1a))
     wgrad(9, d1a, 32, 0, nullptr, 0, g_d1, n, h, w, s);

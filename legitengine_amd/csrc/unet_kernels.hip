@@ -409,13 +409,14 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
 }
 
 // Weight and bias gradient of a 3x3 convolution:
-//   dW[tap][ci][co] += sum_p X[p + shift(tap)][ci] * dY[p][co],  db[co] += sum_p dY[p][co]
+//   dW[tap][ci][co] = sum_p X[p + shift(tap)][ci] * dY[p][co],  db[co] = sum_p dY[p][co]
 // One workgroup: one image, one 32-channel group of the input, a 64-pixel
 // column band and `rows` rows, taken RB rows per round.  Per round the RB + 2
 // input rows and the RB output-gradient rows are staged in LDS transposed
 // (channel-major, written as pixel pairs) so that MFMA operands are 8
-// consecutive pixels of one channel; K runs over pixels.  The horizontally
-// shifted input operands are two aligned 16-byte reads and an element shift.
+// consecutive pixels of one channel; K runs over pixels.  Three aligned
+// 16-byte reads of an input row give the operands of its three horizontal
+// taps (the shifted ones by an element shift).
 // Wave w owns input half cb = w & 1 and the output blocks mb = (w >> 1) + 2j,
 // all 9 taps: per (row, k chunk) it reads each of its A fragments once for
 // the 9 taps and each tap's B fragment once for its MB / 2 blocks.  Each

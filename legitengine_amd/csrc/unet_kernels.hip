@@ -193,6 +193,8 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
 // two kernels give bit-identical results.
 // HEAD: the inference head fused into the epilogue (its own instantiation;
 // needs MB = 2, i.e. all 32 channels of a pixel in one workgroup).
+// With one input chunk and no channel split every tile uses the same
+// weights: they stay in LDS from the workgroup's first tile on.
 template <int MB, int WR, int WP, bool HEAD>
 __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x, int tiles_y, int splits) {
   constexpr int TH = 4 * WR, TW = 16 * WP, IW = TW + 2, IH = TH + 2;
@@ -231,7 +233,8 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
   // serialising the prefetch with the matrix work.
   bf16x8 pin[IN_PER], pw[W_PER];
   uint32_t valid = 0;  // bit i: piece i of pin lies in the image
-  auto fetch = [&](const Tile &tl, int cc) {
+  const bool w_resident = !HEAD && chunks == 1 && splits == 1;
+  auto fetch = [&](const Tile &tl, int cc, bool with_w) {
     const bool from_a = cc * 32 < a.ca;
     const bool up = from_a && a.a_up;
     const int cs = from_a ? a.ca : a.cb;
@@ -249,6 +252,7 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
       pin[i] = *(const bf16x8 *)(img + off);
       valid |= (uint32_t)ok << i;
     }
+    if (!with_w) return;
     const bf16 *wsrc = a.wt + ((size_t)cc * a.cout + tl.co0) * 32;
 #pragma unroll
     for (int i = 0; i < W_PER; i++) {
@@ -260,7 +264,8 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
   int t = blockIdx.x;
   if (t >= T) return;  // the whole workgroup, before any barrier
   Tile cur = tile_of(t);
-  fetch(cur, 0);
+  fetch(cur, 0, true);
+  bool stage_w = true;  // false once resident weights are staged
   for (;;) {
     f32x4 acc[WR][WP][MB];
 #pragma unroll
@@ -276,17 +281,20 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
         const bf16x8 z = {};
         if (q < IN_PIECES) *(bf16x8 *)(s_in + (q & 3) * PLANE_IN + (q >> 2) * 8) = ((valid >> i) & 1u) ? pin[i] : z;
       }
+      if (stage_w) {
 #pragma unroll
-      for (int i = 0; i < W_PER; i++) {
-        const int q = tid + i * 256;
-        if (q < W_PIECES) *(bf16x8 *)(s_w + (q & 3) * PLANE_W + (q >> 2) * 8) = pw[i];
+        for (int i = 0; i < W_PER; i++) {
+          const int q = tid + i * 256;
+          if (q < W_PIECES) *(bf16x8 *)(s_w + (q & 3) * PLANE_W + (q >> 2) * 8) = pw[i];
+        }
+        stage_w = !w_resident;
       }
       __syncthreads();
       // in flight during the matrix work below: the next chunk, or the next tile's first
       if (cc + 1 < chunks)
-        fetch(cur, cc + 1);
+        fetch(cur, cc + 1, true);
       else if (!HEAD && t + (int)gridDim.x < T)  // a head launch has one tile per workgroup
-        fetch(tile_of(t + (int)gridDim.x), 0);
+        fetch(tile_of(t + (int)gridDim.x), 0, !w_resident);
 #pragma unroll
       for (int dx = 0; dx < 3; dx++) {
         bf16x8 bx[WR + 2][WP];

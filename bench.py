@@ -78,8 +78,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build's path runs on a single MI355X "
-              "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
+        print("bench.py measures one GPU only: the GI path runs on a single MI355X "
+              "(sharding across GPUs is described, not measured)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

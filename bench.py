@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-px", type=int, default=96 * 96)
+    ap.add_argument("--cpu-rows-stride", type=int, default=8,
+                    help="CPU baseline: the gather runs on every n-th row (stratified), scaled to the frame")
     ap.add_argument("--no-denoised", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-train", action="store_true")
@@ -70,6 +71,7 @@ def parse():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch a 256 MiB device copy 3x and the 1080p gather 3x (for rocprofv3 --pmc)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_1080p_plain_traffic.md"))
+    ap.add_argument("--counters-file", default=os.path.join(ROOT, "profiles", "gather_1080p_plain_counters.md"))
     ap.add_argument("--quality-seeds", default=",".join(str(s) for s in QUALITY_SEEDS))
     ap.add_argument("--weights", default=None)
     return ap.parse_args()
@@ -106,13 +108,20 @@ def main():
                             denoise_passes=BILAT_PASSES, denoise_radius=BILAT_RADIUS, kind="mixed", profile=True,
                             tile=plain_tile)
 
-    # step counts (outside the timed region) for the algorithmic byte model
+    # step and hit counts (outside the timed region) for the algorithmic byte
+    # model: every executed step reads one depth texel, and only the steps
+    # whose sample rises above the running horizon read a radiance texel
+    # (indirectLighting.frag:254-256); each pixel reads its centre depth and
+    # normal and writes its output once
     counts = torch.zeros((H, W), dtype=torch.int32, device="cuda")
-    dev.Gather(W, H, DIRS, STEPS, sampler)(view, proj, g, pyr, step_counts=counts)
+    hits = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    dev.Gather(W, H, DIRS, STEPS, sampler)(view, proj, g, pyr, step_counts=counts, hit_counts=hits)
     torch.cuda.synchronize()
     total_steps = int(counts.sum().item())
-    del pyr, counts
-    gather_bytes = total_steps * (BD + BR) + W * H * (BC + BO)
+    total_hits = int(hits.sum().item())
+    del pyr, counts, hits
+    gather_bytes = total_steps * BD + total_hits * BR + W * H * (BC + BO)
+    gather_bytes_per_step_model = total_steps * (BD + BR) + W * H * (BC + BO)
 
     for _ in range(args.warmup):
         renderer.render(g, view, proj)
@@ -148,9 +157,16 @@ def main():
                    "width": W, "height": H, "dirs": DIRS, "max_steps": STEPS, "levels": LEVELS, "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "model": "algorithmic bytes at storage precision, no cache credit: depth 4 B per executed "
+                              "step, radiance 8 B per horizon hit, centre 8 B + output 16 B per pixel",
                      "kernel": "gather_kernel (IndirectLightPass)", "kernel_ms": round(gather_ms, 4),
                      "tile": plain_tile, "tile_ms": {k: round(v, 4) for k, v in plain_tiles.items()},
-                     "algorithmic_bytes": gather_bytes, "steps_executed": total_steps},
+                     "algorithmic_bytes": gather_bytes, "steps_executed": total_steps, "hits": total_hits,
+                     "frac_radiance_every_step": round(gather_bytes_per_step_model / (gather_ms * 1e-3) / 1e9
+                                                       / HBM_PEAK_GBS, 4),
+                     "measured_hbm_frac": (round(traffic / (gather_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+                                           if traffic else None),
+                     **read_counters(args.counters_file)},
         "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(passes)},
     }
 
@@ -171,7 +187,7 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(g, view, proj, sampler, renderer.intermediate("indirect"),
-                                              args.cpu_sample_px)
+                                              args.cpu_rows_stride)
     print(json.dumps(result), flush=True)
 
 
@@ -210,6 +226,25 @@ def read_traffic(path):
     except OSError:
         return None
     return None
+
+
+def read_counters(path):
+    """Measured utilisation of the gather from the committed counter summary
+    (lines 'key: value' for valu_busy, valu_utilization, l2_hit,
+    mem_unit_busy, ...), or {}."""
+    out = {}
+    try:
+        for line in open(path):
+            line = line.strip()
+            if line.startswith("counter_") and ":" in line:
+                k, v = line.split(":", 1)
+                try:
+                    out[k.strip()[len("counter_"):]] = float(v.strip())
+                except ValueError:
+                    pass
+    except OSError:
+        return {}
+    return {"measured": out} if out else {}
 
 
 def time_renderer(r, g, view, proj, steps, warmup):
@@ -446,35 +481,58 @@ def batch_leg(args, sampler):
     }
 
 
-def cpu_baseline(g, view, proj, sampler, gpu_gather, sample_px):
-    """FP32 oracle on a bounded sample, fed the GPU's own G-buffer bytes:
-    full-frame pyramids + the gather on a centred crop scaled to the frame +
-    full-frame bilateral passes."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(g, view, proj, sampler, gpu_gather, rows_stride):
+    """The FP32 oracle on the GPU's own G-buffer bytes, the whole plain path:
+    full-frame pyramids and pre-blur, the gather on a stratified sample of
+    whole rows (every `rows_stride`-th row, the offset cycling through the
+    stride so all pattern rows and both borders are covered; its time scaled
+    by H / rows), and the full-frame bilateral passes. One warm-up run, then
+    the median of three."""
     import numpy as np
     from oracle import binding as orc
     threads = orc.lib().orc_max_threads()
     depth = g.depth.numpy()
     octn = orc.widen(g.normal.numpy())
     radiance = orc.widen(g.radiance.numpy())
-    t0 = time.perf_counter()
-    dep_b, rad_b, _, _ = orc.build_pyramids(depth, radiance, LEVELS, sampler, True)
-    t_pyr = time.perf_counter() - t0
-    side = int(np.sqrt(sample_px))
-    x0, y0 = (W - side) // 2, (H - side) // 2
-    t0 = time.perf_counter()
-    ref = orc.gather(view, proj, depth, octn, dep_b, rad_b, DIRS, STEPS, sampler, crop=(x0, y0, side, side))
-    t_gather = (time.perf_counter() - t0) * (W * H) / (side * side)
-    img = np.zeros((H, W, 4), np.float32)
-    t0 = time.perf_counter()
-    for _ in range(BILAT_PASSES):
-        img = orc.bilateral(img, depth, octn, BILAT_RADIUS)
-    t_bil = time.perf_counter() - t0
-    ms = (t_pyr + t_gather + t_bil) * 1e3
-    gpu = gpu_gather[y0:y0 + side, x0:x0 + side]
-    return {"value": round(ms, 1), "unit": "ms/frame", "cores": int(threads), "kind": "port",
-            "sample": f"FP32 oracle on the GPU's G-buffer: full-frame pyramids and {BILAT_PASSES} bilateral "
-                      f"passes, gather on a {side}x{side} centre crop scaled by {W * H / (side * side):.1f}",
-            "gather_rel_l2_on_crop": orc.rel_l2(gpu, ref)}
+    n_rows = (H + rows_stride - 1) // rows_stride
+    rows = sorted({min(H - 1, i * rows_stride + i % rows_stride) for i in range(n_rows)})
+    runs = []
+    for rep in range(4):
+        t0 = time.perf_counter()
+        dep_b, rad_b, _, _ = orc.build_pyramids(depth, radiance, LEVELS, sampler, True)
+        t1 = time.perf_counter()
+        ref, _ = orc.gather_rows(view, proj, depth, octn, dep_b, rad_b, DIRS, STEPS, sampler, rows)
+        t2 = time.perf_counter()
+        img = np.zeros((H, W, 4), np.float32)
+        for _ in range(BILAT_PASSES):
+            img = orc.bilateral(img, depth, octn, BILAT_RADIUS)
+        t3 = time.perf_counter()
+        frame_s = (t1 - t0) + (t2 - t1) * H / len(rows) + (t3 - t2)
+        log(f"cpu baseline run {rep}: {frame_s * 1e3:.0f} ms/frame (gather rows {(t2 - t1) * 1e3:.0f} ms)")
+        if rep > 0:
+            runs.append((frame_s, t1 - t0, (t2 - t1) * H / len(rows), t3 - t2))
+    runs.sort()
+    med = runs[1]
+    return {"value": round(med[0] * 1e3, 1), "unit": "ms/frame", "cores": int(threads), "kind": "port",
+            "cpu_model": cpu_model(), "omp_max_threads": int(threads),
+            "sample": f"FP32 oracle on the GPU's G-buffer, whole plain path: full-frame pyramids and pre-blur, "
+                      f"{BILAT_PASSES} full-frame bilateral passes, gather on {len(rows)} stratified whole rows "
+                      f"(every {rows_stride}th, offset cycling) scaled by {H / len(rows):.2f}; 1 warm-up, "
+                      f"median of 3",
+            "runs_ms": [round(r[0] * 1e3, 1) for r in runs],
+            "split_ms": {"pyramids": round(med[1] * 1e3, 1), "gather": round(med[2] * 1e3, 1),
+                         "bilateral": round(med[3] * 1e3, 1)},
+            "gather_rel_l2_on_rows": orc.rel_l2(gpu_gather[rows], ref)}
 
 
 if __name__ == "__main__":

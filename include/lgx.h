@@ -119,6 +119,8 @@ typedef struct {
                         takes pattern index (x % grid) + (y % grid_y) * grid of grid * grid_y Hammersley
                         offsets (:155-164); grid * grid_y <= 64.  LGX_OUT_SLICES needs the pattern to divide
                         the frame. */
+  int32_t *hit_counts; /* optional device int32[W*H], with step_counts: the steps whose sample rose above the
+                          running horizon (:254), i.e. the ones that fetch radiance, summed over dirs; NULL = off */
 } lgx_gather_params;
 
 /* Workspace for the gather: per-(pattern, direction) screen directions and

@@ -244,6 +244,23 @@ class RenderGraph {
   template <class IdT> friend class Unique;
 };
 
+// legit::ShaderMemoryPool, the per-frame uniform pool the reference's
+// builders and record funcs take (frameInfo.memoryPool, SSVGIRenderer.h:207-221,
+// 238-246).  The lgx passes carry their parameters as kernel arguments, so
+// the pool holds nothing; the type lets call sites written against the
+// reference builders compile unchanged.
+class ShaderMemoryPool {};
+
+// legit::Core (main.cpp:132): owns the render graph the renderers and
+// builders declare passes on (Core::GetRenderGraph).
+class Core {
+ public:
+  RenderGraph *GetRenderGraph() { return &renderGraph_; }
+
+ private:
+  RenderGraph renderGraph_;
+};
+
 template <class IdT> void RenderGraph::Unique<IdT>::reset() {
   if (!g_) return;
   if constexpr (std::is_same<IdT, ImageProxyId>::value) g_->ReleaseImage(id_);

@@ -43,14 +43,30 @@ struct MippedProxy {
   bool externalLevel0;
 };
 
+// The builders take the reference's constructor (legit::Core *) and
+// argument lists, memory pool included (MipBuilder.h:142,
+// BlurBuilder.h:14, InterleaveBuilder.h:14,48), beside shorter forms
+// without the pool.
 class MipBuilder {
  public:
+  MipBuilder() = default;
+  explicit MipBuilder(Core *) {}
   enum struct FilterTypes { Avg, Depth };  // Depth = max-Z reduction
+  void BuildMips(RenderGraph *rg, ShaderMemoryPool *memoryPool, const MippedProxy &proxy,
+                 FilterTypes filter = FilterTypes::Avg) {
+    BuildMips(rg, proxy, filter);
+  }
   void BuildMips(RenderGraph *rg, const MippedProxy &proxy, FilterTypes filter = FilterTypes::Avg);
 };
 
 class BlurBuilder {
  public:
+  BlurBuilder() = default;
+  explicit BlurBuilder(Core *) {}
+  void ApplyBlur(RenderGraph *rg, ShaderMemoryPool *memoryPool, RenderGraph::ImageViewProxyId src,
+                 RenderGraph::ImageViewProxyId dst, int radius) {
+    ApplyBlur(rg, src, dst, radius);
+  }
   void ApplyBlur(RenderGraph *rg, RenderGraph::ImageViewProxyId src, RenderGraph::ImageViewProxyId dst, int radius);
   // one pass blurring every (src[i], dst[i], radius[i])
   void ApplyBlurs(RenderGraph *rg, const std::vector<RenderGraph::ImageViewProxyId> &src,
@@ -59,6 +75,16 @@ class BlurBuilder {
 
 class InterleaveBuilder {
  public:
+  InterleaveBuilder() = default;
+  explicit InterleaveBuilder(Core *) {}
+  void Deinterleave(RenderGraph *rg, ShaderMemoryPool *memoryPool, RenderGraph::ImageViewProxyId interleaved,
+                    RenderGraph::ImageViewProxyId deinterleaved, uvec2 gridSize) {
+    Deinterleave(rg, interleaved, deinterleaved, gridSize);
+  }
+  void Interleave(RenderGraph *rg, ShaderMemoryPool *memoryPool, RenderGraph::ImageViewProxyId deinterleaved,
+                  RenderGraph::ImageViewProxyId interleaved, uvec2 gridSize) {
+    Interleave(rg, deinterleaved, interleaved, gridSize);
+  }
   void Deinterleave(RenderGraph *rg, RenderGraph::ImageViewProxyId interleaved,
                     RenderGraph::ImageViewProxyId deinterleaved, uvec2 gridSize);
   void Interleave(RenderGraph *rg, RenderGraph::ImageViewProxyId deinterleaved,

@@ -66,6 +66,31 @@ def build(verbose=False):
     return LIB
 
 
+EXAMPLE_SRC = os.path.join(ROOT, "tests", "csrc", "integration_example.cpp")
+EXAMPLE_BIN = os.path.join(ROOT, "tests", "csrc", "integration_example")
+
+
+def build_integration_example(verbose=False):
+    """Compiles the C++ consumer of the boundary headers (the code
+    INTEGRATION.md shows) against liblgx.so; the GPU tests run it."""
+    os.makedirs(BUILD, exist_ok=True)
+    obj = os.path.join(BUILD, "integration_example.o")
+    cmd = [HIPCC, *CXX_FLAGS, "-x", "c++", "-c", EXAMPLE_SRC, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    tmp = EXAMPLE_BIN + ".tmp"
+    cmd = [HIPCC, obj, "-o", tmp, "-L", HERE, "-llgx", "-Wl,-rpath,$ORIGIN/../../legitengine_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, EXAMPLE_BIN)
+    if verbose:
+        print(f"built {EXAMPLE_BIN}")
+    return EXAMPLE_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_integration_example(verbose=True)
     sys.exit(0)

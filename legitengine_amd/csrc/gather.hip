@@ -23,6 +23,8 @@
 //    XCD, neighbouring tiles stay on that XCD's L2.
 #include <math.h>
 
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "lgx_internal.h"
@@ -54,6 +56,7 @@ template <class T> struct GatherArgs {
   void *out;
   int out_pitch, out_slices;
   int *step_counts;
+  int *hit_counts;  // COUNT only, may be null
   int tile_w, tile_h, tiles_x, n_blocks;
   int occ;  // host side: which register allocation to launch
   int sx0, sy0, lcells_x, lcells_y;  // slice cells covered by the launch (region of interest)
@@ -195,7 +198,7 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
   const T *dirs = a.dirs + (size_t)k * a.D * 2;
   const T *steps = a.steps + (size_t)k * a.S_tab * 2;
   A sum0 = 0, sum1 = 0, sum2 = 0;
-  int step_total = 0;
+  int step_total = 0, hit_total = 0;
 
   for (int d = 0; d < a.D; d++) {
     const T dx = dirs[2 * d], dy = dirs[2 * d + 1];
@@ -270,6 +273,7 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
           const T hxv = ri * fma_(off, ev, eu) - erc;
           const T pa = pseudo_angle(hyv, hxv);
           if (pa < pa_max) {  // :254
+            if (COUNT) hit_total++;
             const T th = atan2(hyv, hxv);
             const T iq = fast_rcp(hxv * hxv + hyv * hyv);
             const T c2 = (hxv * hxv - hyv * hyv) * iq, s2 = (T(2) * hxv * hyv) * iq;
@@ -320,6 +324,7 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
         const T hxv = ri * fma_(off, ev, eu) - erc;
         const T pa = pseudo_angle(hyv, hxv);
         if (pa < pa_max) {  // :254
+            if (COUNT) hit_total++;
           const T th = atan2(hyv, hxv);
           const T qq = hxv * hxv + hyv * hyv;
           const T c2 = (hxv * hxv - hyv * hyv) / qq, s2 = (T(2) * hxv * hyv) / qq;
@@ -378,7 +383,10 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
     po[2] = (double)sum2;
     po[3] = 1.0;
   }
-  if (COUNT) a.step_counts[(size_t)y * a.w + x] = step_total;
+  if (COUNT) {
+    a.step_counts[(size_t)y * a.w + x] = step_total;
+    if (a.hit_counts) a.hit_counts[(size_t)y * a.w + x] = hit_total;
+  }
 }
 
 // OCC: minimum waves per SIMD the register allocation must allow (1 = the
@@ -471,6 +479,47 @@ static GatherTableShape table_shape(const lgx_gather_params *p) {
   s.S_tab = p->precision == LGX_PREC_F64 ? table_steps<double>(p) : table_steps<float>(p);
   s.elem = p->precision == LGX_PREC_F64 ? sizeof(double) : sizeof(float);
   return s;
+}
+
+// What a workspace's tables were computed for: every parameter the tables
+// depend on, normalised (grid_y <= 0 is grid, footprint <= 0 is dirs; F32 and
+// F32_ACC64 share the float tables).
+struct TableKey {
+  int32_t width, height, dirs, max_steps, grid, grid_y, footprint, f64;
+  bool operator==(const TableKey &o) const {
+    return width == o.width && height == o.height && dirs == o.dirs && max_steps == o.max_steps && grid == o.grid &&
+           grid_y == o.grid_y && footprint == o.footprint && f64 == o.f64;
+  }
+};
+
+static TableKey table_key(const lgx_gather_params *p) {
+  TableKey k;
+  k.width = p->width;
+  k.height = p->height;
+  k.dirs = p->dirs;
+  k.max_steps = p->max_steps > 0 ? p->max_steps : 0;
+  k.grid = p->grid;
+  k.grid_y = p->grid_y > 0 ? p->grid_y : p->grid;
+  k.footprint = p->footprint_dirs > 0 ? p->footprint_dirs : p->dirs;
+  k.f64 = p->precision == LGX_PREC_F64;
+  return k;
+}
+
+// Workspaces filled by lgx_gather_prepare, by device address.  lgx_gather
+// and lgx_gather_batch_create reject a workspace that was not prepared for
+// the call's parameters (a smaller table would be read past its end).
+static std::mutex g_ws_mu;
+static std::unordered_map<const void *, TableKey> g_ws;
+
+static void record_workspace(const void *ws, const lgx_gather_params *p) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  g_ws[ws] = table_key(p);
+}
+
+static bool workspace_prepared(const void *ws, const lgx_gather_params *p) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto it = g_ws.find(ws);
+  return it != g_ws.end() && it->second == table_key(p);
 }
 
 // Tables in the shader's own arithmetic and evaluation order
@@ -608,7 +657,9 @@ extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
   std::vector<unsigned char> t;
   bool ok = p->precision == LGX_PREC_F64 ? fill_tables<double>(p, s, t) : fill_tables<float>(p, s, t);
   if (!ok) return LGX_EUNSUPPORTED;
-  return hip_status(hipMemcpy(workspace, t.data(), t.size(), hipMemcpyHostToDevice));
+  const int rc = hip_status(hipMemcpy(workspace, t.data(), t.size(), hipMemcpyHostToDevice));
+  if (rc == LGX_OK) record_workspace(workspace, p);
+  return rc;
 }
 
 extern "C" int lgx_gather_tiles(void) { return kNumTiles; }
@@ -655,6 +706,7 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
   a.out_pitch = out->pitch;
   a.out_slices = p->out_layout == LGX_OUT_SLICES;
   a.step_counts = p->step_counts;
+  a.hit_counts = p->step_counts ? p->hit_counts : nullptr;
   int tile = p->tile;
   if (tile < 0 || tile >= kNumTiles) return LGX_EUNSUPPORTED;
   a.tile_w = kTiles[tile][0];
@@ -695,11 +747,10 @@ static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbu
   return hip_status(e);
 }
 
-// The argument checks of lgx_gather.
-static int check_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
-                        const lgx_pyramid *pyr, const lgx_image *out) {
+// The argument checks of lgx_gather, apart from the workspace.
+static int check_gather_images(const lgx_gather_params *p, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+                               const lgx_image *out) {
   if (!gather_params_ok(p) || !g || !pyr || !out) return LGX_EINVAL;
-  if (!workspace) return LGX_EWORKSPACE;
   if (!image_ok(&g->depth) || !image_ok(&g->normal) || !image_ok(out)) return LGX_EINVAL;
   if (g->depth.format != LGX_R32F) return LGX_EFORMAT;
   if (g->normal.format != LGX_RG16F && g->normal.format != LGX_RG32F) return LGX_EFORMAT;
@@ -726,6 +777,16 @@ static int check_gather(const lgx_gather_params *p, const void *workspace, const
     eh /= 2;
   }
   return LGX_OK;
+}
+
+// The argument checks of lgx_gather.
+static int check_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
+                        const lgx_pyramid *pyr, const lgx_image *out) {
+  if (!gather_params_ok(p) || !g || !pyr || !out) return LGX_EINVAL;
+  if (!workspace) return LGX_EWORKSPACE;
+  const int rc = check_gather_images(p, g, pyr, out);
+  if (rc != LGX_OK) return rc;
+  return workspace_prepared(workspace, p) ? LGX_OK : LGX_EWORKSPACE;
 }
 
 extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
@@ -806,17 +867,17 @@ extern "C" int lgx_gather_batch_create(const lgx_gather_params *p, const void *c
                                        lgx_gather_batch **res) {
   if (!p || !workspaces || !g || !pyr || !out || !res || n < 1) return LGX_EINVAL;
   *res = nullptr;
+  if (p[0].tile < 0 || p[0].tile >= kNumTiles) return LGX_EUNSUPPORTED;
   for (int i = 0; i < n; i++) {
     int rc = check_gather(&p[i], workspaces[i], &g[i], &pyr[i], &out[i]);
     if (rc != LGX_OK) return rc;
-    if (p[i].precision != LGX_PREC_F32 || p[i].step_counts) return LGX_EUNSUPPORTED;
+    if (p[i].precision != LGX_PREC_F32 || p[i].step_counts || p[i].hit_counts) return LGX_EUNSUPPORTED;
     if (p[i].sampler != p[0].sampler || p[i].tile != p[0].tile ||
         pyr[i].radiance[0].format != pyr[0].radiance[0].format || g[i].normal.format != g[0].normal.format)
       return LGX_EINVAL;
   }
   const int rf = pyr[0].radiance[0].format, nf = g[0].normal.format;
   if (!((rf == LGX_RGBA16F && nf == LGX_RG16F) || (rf == LGX_RGBA32F && nf == LGX_RG32F))) return LGX_EFORMAT;
-  if (p[0].tile < 0 || p[0].tile >= kNumTiles) return LGX_EUNSUPPORTED;
   lgx_gather_batch *b = new (std::nothrow) lgx_gather_batch;
   if (!b) return LGX_EINVAL;
   b->p.assign(p, p + n);
@@ -864,3 +925,4 @@ extern "C" int lgx_gather_batch_launch(const lgx_gather_batch *b, int32_t workgr
 }
 
 extern "C" void lgx_gather_batch_destroy(lgx_gather_batch *b) { delete b; }
+

@@ -563,10 +563,17 @@ extern "C" int lgx_unet_set_state(lgx_unet *u, int32_t which, const float *host,
 extern "C" int lgx_unet_denoise(lgx_unet *u, const lgx_gbuffer *g, const lgx_image *noisy, lgx_image *out,
                                 lgx_stream_t stream) {
   if (!u || !u->net || !g || !noisy || !out) return LGX_EINVAL;
-  if (noisy->format != LGX_RGBA32F || out->format != LGX_RGBA32F) return LGX_EFORMAT;
-  if (out->w != noisy->w || out->h != noisy->h || g->depth.w != noisy->w || g->depth.h != noisy->h ||
-      g->normal.w != noisy->w || g->albedo.w != noisy->w)
-    return LGX_ESIZE;
+  // every image the packing and the fused head touch, checked before any launch
+  using lgx::image_ok;
+  if (!image_ok(noisy) || !image_ok(out) || !image_ok(&g->depth) || !image_ok(&g->normal) || !image_ok(&g->albedo))
+    return LGX_EINVAL;
+  if (noisy->format != LGX_RGBA32F || out->format != LGX_RGBA32F || g->depth.format != LGX_R32F) return LGX_EFORMAT;
+  if (!((g->normal.format == LGX_RG16F && g->albedo.format == LGX_RGBA8) ||
+        (g->normal.format == LGX_RG32F && g->albedo.format == LGX_RGBA32F)))
+    return LGX_EFORMAT;
+  const lgx_image *all[4] = {out, &g->depth, &g->normal, &g->albedo};
+  for (const lgx_image *im : all)
+    if (im->w != noisy->w || im->h != noisy->h) return LGX_ESIZE;
   if (!shape_ok(u, 1, noisy->h, noisy->w)) return LGX_ESIZE;
   return unet_guard([&]() {
     Net &N = *u->net;

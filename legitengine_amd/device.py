@@ -147,12 +147,13 @@ class Gather:
         lgx.check(lgx.lib().lgx_gather_prepare(C.byref(p), C.c_void_p(self.ws.data_ptr())), "lgx_gather_prepare")
         self.out_fmt = lgx.RGBA32F if precision == lgx.PREC_F32 else lgx.RGBA64F
 
-    def __call__(self, view, proj, gb, pyr, out=None, step_counts=None, stream=None, roi=None):
+    def __call__(self, view, proj, gb, pyr, out=None, step_counts=None, stream=None, roi=None, hit_counts=None):
         p = self.p
         p.roi[:] = list(roi) if roi is not None else [0, 0, 0, 0]
         p.view[:] = [float(v) for v in view]
         p.proj[:] = [float(v) for v in proj]
         p.step_counts = C.c_void_p(step_counts.data_ptr()) if step_counts is not None else None
+        p.hit_counts = C.c_void_p(hit_counts.data_ptr()) if hit_counts is not None else None
         if out is None:
             out = DImage(p.width, p.height, self.out_fmt)
         g, pd, o = gb.desc, pyr.desc, out.desc
@@ -393,6 +394,7 @@ class GatherBatch:
             ps[i].proj[:] = [float(v) for v in proj]
             ps[i].roi[:] = [0, 0, 0, 0]
             ps[i].step_counts = None
+            ps[i].hit_counts = None
         ws = (C.c_void_p * n)(*[it[0].ws.data_ptr() for it in items])
         gs = (lgx.GBuffer * n)(*[it[1].desc for it in items])
         pyrs = (lgx.Pyramid * n)(*[it[2].desc for it in items])

@@ -45,7 +45,8 @@ class GatherParams(C.Structure):
     _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("width", C.c_int32), ("height", C.c_int32),
                 ("dirs", C.c_int32), ("max_steps", C.c_int32), ("grid", C.c_int32), ("sampler", C.c_int32),
                 ("precision", C.c_int32), ("out_layout", C.c_int32), ("step_counts", C.c_void_p),
-                ("tile", C.c_int32), ("roi", C.c_int32 * 4), ("footprint_dirs", C.c_int32), ("grid_y", C.c_int32)]
+                ("tile", C.c_int32), ("roi", C.c_int32 * 4), ("footprint_dirs", C.c_int32), ("grid_y", C.c_int32),
+                ("hit_counts", C.c_void_p)]
 
 
 class BilateralParams(C.Structure):

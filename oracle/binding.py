@@ -42,6 +42,8 @@ def lib():
                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.orc_gather_f64.argtypes = [C.POINTER(GatherParams), FP, FP, C.POINTER(FP), C.POINTER(FP), IP, IP, DP, IP,
                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_gather_rows_f32.argtypes = [C.POINTER(GatherParams), FP, FP, C.POINTER(FP), C.POINTER(FP), IP, IP, IP,
+                                          C.c_int32, FP, IP, IP, C.c_int32]
         L.orc_mip_avg.argtypes = [FP, C.c_int32, C.c_int32, C.c_int32, FP, C.c_int32]
         L.orc_mip_max.argtypes = [FP, C.c_int32, C.c_int32, FP]
         L.orc_box_blur.argtypes = [FP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, FP, C.c_int32]
@@ -176,6 +178,35 @@ def gather(view, proj, depth, oct_normal, pyr_depth, pyr_rad, dirs, max_steps, s
                                   steps.ctypes.data_as(IP), x0, y0, cw, ch, threads)
     assert rc == 0, rc
     return (out, steps) if with_steps else out
+
+
+def gather_rows(view, proj, depth, oct_normal, pyr_depth, pyr_rad, dirs, max_steps, sampler, rows, grid=4,
+                threads=0, footprint_dirs=0, grid_y=0, with_hits=False):
+    """FP32 gather of whole rows (a stratified sample of the frame): returns
+    (out [len(rows), W, 4], steps [len(rows), W]) and, with_hits, the
+    per-pixel count of steps that fetched radiance."""
+    h, w = depth.shape
+    p = GatherParams()
+    p.footprint_dirs = footprint_dirs
+    p.view[:] = [float(v) for v in view]
+    p.proj[:] = [float(v) for v in proj]
+    gx, gy = (grid, grid_y) if isinstance(grid, int) else grid
+    p.width, p.height, p.dirs, p.max_steps, p.grid, p.grid_y, p.sampler = w, h, dirs, max_steps, gx, gy, sampler
+    p.n_levels = len(pyr_depth)
+    pd = (FP * len(pyr_depth))(*[fp(a) for a in pyr_depth])
+    pr = (FP * len(pyr_rad))(*[fp(a) for a in pyr_rad])
+    lw = (C.c_int32 * len(pyr_depth))(*[a.shape[1] for a in pyr_depth])
+    lh = (C.c_int32 * len(pyr_depth))(*[a.shape[0] for a in pyr_depth])
+    r = np.ascontiguousarray(rows, np.int32)
+    out = np.zeros((len(r), w, 4), np.float32)
+    steps = np.zeros((len(r), w), np.int32)
+    hits = np.zeros((len(r), w), np.int32)
+    rc = lib().orc_gather_rows_f32(C.byref(p), fp(np.ascontiguousarray(depth, np.float32)),
+                                   fp(np.ascontiguousarray(oct_normal, np.float32)), pd, pr, lw, lh,
+                                   r.ctypes.data_as(IP), len(r), fp(out), steps.ctypes.data_as(IP),
+                                   hits.ctypes.data_as(IP), threads)
+    assert rc == 0, rc
+    return (out, steps, hits) if with_hits else (out, steps)
 
 
 def bilateral(inp, depth, oct_normal, radius=4, ds=10.5, ns=20.0, threads=0):

@@ -288,7 +288,8 @@ int nthreads(int t) {
  * direction from the camera enters the tangent (:183). */
 template <class T>
 void gather_pixel(const orc_gather_params &p, const M4<T> &invVP, V3<T> cam, int px, int py, const float *depth,
-                  const float *oct, const Level *ld, const Level *lr, T *out_rgb, int32_t *steps_out) {
+                  const float *oct, const Level *ld, const Level *lr, T *out_rgb, int32_t *steps_out,
+                  int32_t *hits_out = nullptr) {
   const T W = T(p.width), H = T(p.height);
   const int D = p.dirs;
   V2<T> c{T(px) + T(0.5), T(py) + T(0.5)}; /* gl_FragCoord.xy, :116 */
@@ -315,7 +316,7 @@ void gather_pixel(const orc_gather_params &p, const M4<T> &invVP, V3<T> cam, int
   const T ratio = T(2) * pi / Df + T(1);
 
   T sum[3] = {0, 0, 0};
-  int32_t steps_total = 0;
+  int32_t steps_total = 0, hits_total = 0;
   for (int d = 0; d < D; d++) { /* :174 */
     T screen_ang = pixel_ang_offset + T(2) * pi / T(D) * T(d); /* :177 */
     V2<T> dir{std::cos(screen_ang), std::sin(screen_ang)};
@@ -364,6 +365,7 @@ void gather_pixel(const orc_gather_params &p, const M4<T> &invVP, V3<T> cam, int
       V3<T> delta = sp - P;                                                                /* :249 */
       T sh = std::atan2(dot(tangent, delta), dot(eye, delta));                            /* :250-252 */
       if (sh < max_h) {                                                                    /* :254 */
+        if (side > T(0)) hits_total++; /* an on-screen step that fetches radiance (:256) */
         T ls[4];
         if (p.sampler == 0)
           sample_trilinear<T>(lr, p.n_levels, 4, sq.x, sq.y, lod, ls); /* :256 */
@@ -381,6 +383,7 @@ void gather_pixel(const orc_gather_params &p, const M4<T> &invVP, V3<T> cam, int
   }
   for (int k = 0; k < 3; k++) out_rgb[k] = sum[k];
   if (steps_out) *steps_out = steps_total;
+  if (hits_out) *hits_out = hits_total;
 }
 
 template <class T, class O>
@@ -416,6 +419,48 @@ int gather(const orc_gather_params *p, const float *depth, const float *oct, con
       out[o * 4 + 2] = O(rgb[2]);
       out[o * 4 + 3] = O(1); /* :270 */
       if (steps) steps[o] = st;
+    }
+  }
+  return 0;
+}
+
+/* The gather on a list of whole rows (stratified samples of a frame);
+ * out holds n_rows x width RGBA in list order. */
+template <class T, class O>
+int gather_rows(const orc_gather_params *p, const float *depth, const float *oct, const float *const *pd,
+                const float *const *pr, const int32_t *lw, const int32_t *lh, const int32_t *rows, int n_rows,
+                O *out, int32_t *steps, int32_t *hits, int threads) {
+  if (!p || !rows || n_rows < 1 || !depth || !oct || !pd || !pr || !out || p->n_levels < 1 || p->n_levels > 16 ||
+      p->dirs < 1 || p->grid < 1)
+    return -1;
+  for (int i = 0; i < n_rows; i++)
+    if (rows[i] < 0 || rows[i] >= p->height) return -2;
+  Level ld[16], lr[16];
+  for (int l = 0; l < p->n_levels; l++) {
+    ld[l] = Level{pd[l], lw[l], lh[l]};
+    lr[l] = Level{pr[l], lw[l], lh[l]};
+  }
+  M4<T> view = load<T>(p->view), proj = load<T>(p->proj);
+  M4<T> invVP = inverse(mul(proj, view)); /* :123-124 */
+  M4<T> invV = inverse(view);             /* :125 */
+  V4<T> cam4 = mul(invV, V4<T>{0, 0, 0, 1}); /* :128 */
+  V3<T> cam{cam4.x, cam4.y, cam4.z};
+  const int w = p->width;
+#pragma omp parallel for schedule(dynamic) collapse(2) num_threads(nthreads(threads))
+  for (int i = 0; i < n_rows; i++) {
+    for (int xb = 0; xb < (w + 63) / 64; xb++) {
+      for (int x = xb * 64; x < std::min(w, xb * 64 + 64); x++) {
+        T rgb[3];
+        int32_t st = 0, hi = 0;
+        gather_pixel<T>(*p, invVP, cam, x, rows[i], depth, oct, ld, lr, rgb, &st, &hi);
+        size_t o = (size_t)i * w + x;
+        out[o * 4 + 0] = O(rgb[0]);
+        out[o * 4 + 1] = O(rgb[1]);
+        out[o * 4 + 2] = O(rgb[2]);
+        out[o * 4 + 3] = O(1);
+        if (steps) steps[o] = st;
+        if (hits) hits[o] = hi;
+      }
     }
   }
   return 0;
@@ -582,6 +627,11 @@ int orc_gather_f32(const orc_gather_params *p, const float *depth, const float *
                    const float *const *pr, const int32_t *lw, const int32_t *lh, float *out, int32_t *steps,
                    int32_t x0, int32_t y0, int32_t cw, int32_t ch, int32_t threads) {
   return gather<float, float>(p, depth, oct, pd, pr, lw, lh, out, steps, x0, y0, cw, ch, threads);
+}
+int orc_gather_rows_f32(const orc_gather_params *p, const float *depth, const float *oct, const float *const *pd,
+                        const float *const *pr, const int32_t *lw, const int32_t *lh, const int32_t *rows,
+                        int32_t n_rows, float *out, int32_t *steps, int32_t *hits, int32_t threads) {
+  return gather_rows<float, float>(p, depth, oct, pd, pr, lw, lh, rows, n_rows, out, steps, hits, threads);
 }
 int orc_gather_f64(const orc_gather_params *p, const float *depth, const float *oct, const float *const *pd,
                    const float *const *pr, const int32_t *lw, const int32_t *lh, double *out, int32_t *steps,

@@ -79,6 +79,16 @@ int orc_gather_f64(const orc_gather_params *p, const float *depth, const float *
                    const int32_t *lw, const int32_t *lh, double *out, int32_t *steps,
                    int32_t x0, int32_t y0, int32_t cw, int32_t ch, int32_t threads);
 
+/* The FP32 gather on whole rows rows[0..n_rows) (any order, repeats allowed):
+ * out = n_rows x width RGBA in list order; steps and hits (optional)
+ * likewise, hits = on-screen steps whose sample rose above the running
+ * horizon (indirectLighting.frag:254) and so fetched radiance.  For
+ * stratified samples of a frame: OpenMP spreads 64-pixel row segments. */
+int orc_gather_rows_f32(const orc_gather_params *p, const float *depth, const float *oct,
+                        const float *const *pyr_depth, const float *const *pyr_rad,
+                        const int32_t *lw, const int32_t *lh, const int32_t *rows, int32_t n_rows,
+                        float *out, int32_t *steps, int32_t *hits, int32_t threads);
+
 /* Geometry-aware bilateral (denoiser.frag:87-110 weights), taps [-r, r)^2. */
 int orc_bilateral(const float *in, const float *depth, const float *oct, int32_t w, int32_t h,
                   int32_t radius, float depth_sigma, float normal_sigma, float *out, int32_t threads);

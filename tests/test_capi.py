@@ -115,12 +115,15 @@ def test_batch_argument_checks():
     pyr.radiance[0] = lgx.image(fake, w, hh, lgx.RGBA16F)
     out = lgx.image(fake, w, hh, lgx.RGBA32F)
     ws = (C.c_void_p * 1)(fake)
-    for field, value, code in (("precision", lgx.PREC_F64, lgx.EFORMAT), ("tile", 99, lgx.EUNSUPPORTED)):
+    for field, value, code in (("precision", lgx.PREC_F64, lgx.EFORMAT), ("tile", 99, lgx.EUNSUPPORTED),
+                               ("dirs", 4, lgx.EWORKSPACE)):
         q = lgx.GatherParams()
         C.memmove(C.byref(q), C.byref(p), C.sizeof(q))
         setattr(q, field, value)
         rc = L.lgx_gather_batch_create(C.byref(q), ws, C.byref(g), C.byref(pyr), C.byref(out), 1, C.byref(h))
         assert rc == code and not h.value, (field, rc)
+    # a workspace lgx_gather_prepare never filled is rejected before any launch
+    assert L.lgx_gather(C.byref(p), C.c_void_p(fake), C.byref(g), C.byref(pyr), C.byref(out), None) == lgx.EWORKSPACE
 
 
 @pytest.fixture(scope="module")

@@ -190,3 +190,67 @@ def test_register_tiles_bit_identical_and_autotune(dev):
     assert [r["best"] for r in rep["forward"]] == f
     assert [r["best"] for r in rep["input_gradient"]] == d[1:]
     assert net.forward(x).cpu().numpy().tobytes() == ref_out.tobytes()
+
+
+def test_fused_head_every_tile(dev):
+    """The inference head runs in the epilogue of the last convolution's LDS
+    tile (the autotuned one, or the first LDS tile when the chosen tile has
+    no LDS form). Every tile of that layer gives the unfused forward's
+    output."""
+    from legitengine_amd import lgx
+    sc = lgx.make_scene(6, 8, 2)
+    gb = lgx.render_gbuffer_cpu(sc, 80, 48, "mixed")
+    g = dev.GBufferDev.from_host(gb)
+    noisy = np.random.default_rng(2).random((48, 80, 4)).astype(np.float32)
+    dn = dev.DImage.from_numpy(noisy, lgx.RGBA32F)
+    net = dev.UNet(1, 48, 80, seed=4)
+    rec = np.zeros((1, 48, 80, 12), np.float32)
+    rec[0, ..., :3] = noisy[..., :3]
+    rec[0, ..., 3] = gb["depth"]
+    rec[0, ..., 4:6] = gb["normal"].astype(np.float32)
+    rec[0, ..., 6:9] = gb["albedo"][..., :3].astype(np.float32) * np.float32(1.0 / 255.0)
+    ref = net.forward(torch.from_numpy(rec).cuda()).cpu().numpy()[0]
+    f0, d0 = net.tiles()
+    n9 = lgx.lib().lgx_unet_conv_tiles(32)
+    assert n9 >= 2
+    for t in range(n9):
+        f = list(f0)
+        f[9] = t
+        net.set_tiles(f, d0)
+        out = net.denoise(g, dn).numpy()
+        assert np.allclose(out, ref, rtol=0, atol=1e-6), (t, np.abs(out - ref).max())
+
+
+def test_denoise_rejects_bad_images_on_the_host(dev):
+    """lgx_unet_denoise checks every image it reads or writes before any
+    launch: null or unpitched images, wrong formats, and extents that differ
+    from the noisy image (the descriptors below are never dereferenced)."""
+    import ctypes as C
+    from legitengine_amd import lgx
+    L = lgx.lib()
+    net = dev.UNet(1, 32, 32, seed=1)
+    fake = 0x100000
+    w, h = 32, 32
+
+    def gbuf(nf=lgx.RG16F, af=lgx.RGBA8, dh=h, nh=h, ah=h, df=lgx.R32F, null_normal=False):
+        g = lgx.GBuffer()
+        g.depth = lgx.image(fake, w, dh, df)
+        g.normal = lgx.image(0 if null_normal else fake, w, nh, nf)
+        g.albedo = lgx.image(fake, w, ah, af)
+        g.radiance = lgx.image(fake, w, h, lgx.RGBA16F)
+        return g
+
+    noisy = lgx.image(fake, w, h, lgx.RGBA32F)
+    out = lgx.image(fake, w, h, lgx.RGBA32F)
+    cases = [
+        (gbuf(nh=h - 4), lgx.ESIZE), (gbuf(ah=h - 4), lgx.ESIZE), (gbuf(dh=h - 4), lgx.ESIZE),
+        (gbuf(df=lgx.RG32F), lgx.EFORMAT), (gbuf(nf=lgx.RG32F), lgx.EFORMAT),
+        (gbuf(null_normal=True), lgx.EINVAL),
+    ]
+    for g, code in cases:
+        assert L.lgx_unet_denoise(net.h, C.byref(g), C.byref(noisy), C.byref(out), None) == code
+    bad = lgx.image(fake, w, h, lgx.RGBA32F, pitch=w - 1)
+    g = gbuf()
+    assert L.lgx_unet_denoise(net.h, C.byref(g), C.byref(noisy), C.byref(bad), None) == lgx.EINVAL
+    small = lgx.image(fake, w, h - 4, lgx.RGBA32F)
+    assert L.lgx_unet_denoise(net.h, C.byref(g), C.byref(noisy), C.byref(small), None) == lgx.ESIZE

@@ -48,3 +48,17 @@ def test_batch_resolutions_are_seeded():
     assert sizes == [bench.batch_resolution(s) for s in bench.BATCH_SEEDS]
     assert set(sizes) <= set(bench.BATCH_SIZES) and len(set(sizes)) == 4
     assert all(w % 4 == 0 and h % 4 == 0 for w, h in sizes)  # the U-Net's two 2x poolings
+
+
+def test_integration_doc_is_the_compiled_example():
+    """INTEGRATION.md section 1's C++ is the code between the snippet markers
+    of tests/csrc/integration_example.cpp, which build() compiles."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = open(os.path.join(root, "tests", "csrc", "integration_example.cpp")).read()
+    snippet = src.split("// [integration-snippet-begin]\n", 1)[1].split("    // [integration-snippet-end]", 1)[0]
+    doc = open(os.path.join(root, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```cpp\n(.*?)```", doc, flags=re.S)
+    dedent = "\n".join(line[4:] if line.startswith("    ") else line for line in snippet.splitlines()) + "\n"
+    assert any(b == dedent for b in blocks), "INTEGRATION.md does not hold the compiled snippet"

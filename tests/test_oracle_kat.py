@@ -13,6 +13,8 @@ import pytest
 
 from oracle import binding as orc
 
+from tests import kat_scenes as ks
+
 
 def c_arr(a, ct=None):
     import ctypes as C
@@ -520,3 +522,89 @@ def test_regression_denoise_affine():
     out = orc.regression_denoise(col, depth, 2)
     assert np.allclose(out[..., :3], col[..., :3], atol=2e-4)
     assert np.array_equal(orc.regression_denoise(col, depth, 0), col)
+
+
+# ---------------------------------------------------------------- gather hit branch (closed form)
+
+def _sphere_gather(sc, dirs, fdirs, precision, pixels):
+    out = orc.gather(sc["view"], sc["proj"], sc["depth"], sc["oct"], [sc["depth"]], [sc["rad"]], dirs, 0, 1,
+                     grid=1, precision=precision, footprint_dirs=fdirs)
+    return np.array([out[y, x, :3] for x, y in pixels])
+
+
+SPHERE_PIXELS = [(x, y) for y in (7, 15, 22, 30, 37, 45, 52) for x in (9, 20, 33, 40, 47, 58, 70)]
+
+
+def test_sphere_scene_every_sample_rises():
+    """Precondition of the closed form: along every direction the on-screen
+    samples' horizon angles fall strictly below pi/2 and keep falling, so each
+    sample is a hit (:254) and the hits come in marching order."""
+    sc = ks.sphere_scene()
+    for x, y in SPHERE_PIXELS[::5]:
+        for th in ks.slice_angles(sc, x, y, 8):
+            th = np.array(th)
+            assert len(th) > 3 and (th < math.pi / 2).all() and (np.diff(th) < 0).all()
+
+
+@pytest.mark.parametrize("dirs,fdirs", [(8, 16), (4, 32), (16, 64)])
+def test_gather_hit_branch_telescopes(dirs, fdirs):
+    """Every sample is a hit. Ring texels hold the ambient radiance, so ring
+    hits add nothing, and each direction's hit sum telescopes to
+      (L - ambient) h(theta_last_interior, pi/2).
+    Each direction gives ambient h(0, pi/2) + (L - ambient) h(theta_last, pi/2).
+    The result depends on the contribution's orientation h(theta, theta_max)
+    (:258), the subtraction of the ambient part (:262) and the horizon update
+    (:263); a flipped or dropped term moves it by far more than the tolerance."""
+    sc = ks.sphere_scene(ring_ambient=True)
+    got = _sphere_gather(sc, dirs, fdirs, "f64", SPHERE_PIXELS)
+    exp = np.array([ks.sphere_expected(sc, x, y, dirs, fdirs) for x, y in SPHERE_PIXELS])
+    ambient_only = ks.AMBIENT * 1.0  # h(0, pi/2) sums to 1/2 per slice, 2 sum / D = 1 for n = omega0
+    assert (exp > 3 * ambient_only).all()  # the hit branch carries most of the result
+    assert np.allclose(got, exp[:, None], rtol=0, atol=2e-6), np.abs(got - exp[:, None]).max()
+    got32 = _sphere_gather(sc, dirs, fdirs, "f32", SPHERE_PIXELS)
+    assert np.allclose(got32, exp[:, None], rtol=2e-4, atol=0), np.abs(got32 / exp[:, None] - 1).max()
+
+
+def test_gather_hit_branch_side_weights():
+    """Uniform radiance: ring hits count with the side fade (:220-228,
+    :258), hit by hit. The closed form sums (L - ambient) side_j
+    h(theta_j, theta_j-1) over the samples."""
+    sc = ks.sphere_scene(ring_ambient=False)
+    got = _sphere_gather(sc, 8, 16, "f64", SPHERE_PIXELS)
+    exp = np.array([ks.sphere_expected(sc, x, y, 8, 16) for x, y in SPHERE_PIXELS])
+    assert np.allclose(got, exp[:, None], rtol=0, atol=2e-6), np.abs(got - exp[:, None]).max()
+    # the side weights matter: without them the ring pixels would differ
+    flat = dict(sc, ring_ambient=True)
+    tele = np.array([ks.sphere_expected(flat, x, y, 8, 16) for x, y in SPHERE_PIXELS])
+    assert np.abs(exp - tele).max() > 1e-3
+
+
+# ---------------------------------------------------------------- bilateral weights (closed form)
+
+def _decode(octn):
+    import ctypes as C
+    out = np.zeros(octn.shape[:-1] + (3,), np.float32)
+    buf = (C.c_float * 3)()
+    for idx in np.ndindex(octn.shape[:-1]):
+        orc.lib().orc_oct_decode(float(octn[idx][0]), float(octn[idx][1]), buf)
+        out[idx] = list(buf)
+    return out
+
+
+@pytest.mark.parametrize("radius", [1, 2, 4])
+def test_bilateral_two_region_known_answer(radius):
+    """Two regions with different depth and normal. A cross tap weighs
+    exp(-10.5 |dd|) exp(-20 |dn|^2) (denoiser.frag:102-105), with dd = 0.05
+    and |dn|^2 = 0.04. That weight (0.266) differs from the swapped
+    constants (0.242) and from exp(-20 |dn|) (0.011), and every output
+    column is a closed-form ratio of tap counts."""
+    depth, octn, col = ks.two_region_field()
+    got = orc.bilateral(col, depth, octn, radius)
+    exp, wc = ks.bilateral_expected(depth, octn, col, radius, split=10, decode=_decode)
+    assert abs(wc - math.exp(-10.5 * 0.05 - 20 * 0.04)) < 2e-6
+    assert np.allclose(got[..., :3], exp, rtol=0, atol=3e-6), np.abs(got[..., :3] - exp).max()
+    # hand-worked value: radius 1, first right column (x = 10): taps x = 9 (left) and 10 (right)
+    if radius == 1:
+        w = math.exp(-0.525) * math.exp(-0.8)
+        assert got[5, 10, 0] == pytest.approx((2 * 0.8 + 2 * w * 0.2) / (2 + 2 * w + 1e-7), abs=3e-6)
+        assert got[5, 9, 0] == pytest.approx(0.2, abs=1e-7)  # taps x = 8, 9: own region only

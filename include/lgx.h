@@ -137,6 +137,22 @@ int lgx_gather_prepare(const lgx_gather_params *p, void *workspace);
 int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g, const lgx_pyramid *pyr,
                lgx_image *out, lgx_stream_t stream);
 
+/* IndirectLightPass without allocations: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Computes
+ * exactly what lgx_gather computes, with the workspace tables held in a
+ * static device array of LGX_GATHER_SCRATCH_BYTES per GPU (kept until the
+ * process ends) instead of caller memory.  Rejects parameters whose tables
+ * exceed it with LGX_EUNSUPPORTED (only large FP64 table sets do).  The
+ * first call with a given set of table parameters (extent, dirs, max_steps,
+ * pattern, footprint, FP64 or not) on a device uploads the tables
+ * synchronously on `stream`; later calls with the same set launch only the
+ * kernel.  Calls that overlap in time on one GPU with different table
+ * parameters race on the shared array. */
+#define LGX_GATHER_SCRATCH_BYTES (512 * 1024)
+int lgx_gather_noalloc(const lgx_gather_params *p, const lgx_gbuffer *g, const lgx_pyramid *pyr, lgx_image *out,
+                       lgx_stream_t stream);
+
 /* MipBuilder::BuildMips (MipBuilder.h:142-181) + mipLevelBuilder.frag.
  * levels[0] is the source; levels[1..n_levels-1] are written, each
  * floor(prev/2) in both extents.  AVG: mean of the 2x2 children in the

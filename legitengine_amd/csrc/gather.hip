@@ -926,3 +926,68 @@ extern "C" int lgx_gather_batch_launch(const lgx_gather_batch *b, int32_t workgr
 
 extern "C" void lgx_gather_batch_destroy(lgx_gather_batch *b) { delete b; }
 
+// ---------------------------------------------------------------- the gather without allocations
+//
+// lgx_gather_noalloc is lgx_gather with its workspace (the direction, step
+// and level tables) held in `scratch`, a file-scope device array, instead of
+// caller memory.  The array is read through the address hipGetSymbolAddress
+// returns for the current device (its host-side name is not a device
+// address), once per device on the first call there.  The tables are
+// uploaded when the call's table parameters differ from the last upload on
+// that device; a call whose tables are already resident launches only the
+// gather kernel.  Every call on a device shares its array.
+static __device__ __align__(256) unsigned char scratch[LGX_GATHER_SCRATCH_BYTES];
+
+namespace lgx {
+
+struct ScratchSlot {
+  void *ptr = nullptr;
+  bool loaded = false;
+  TableKey key{};
+};
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static ScratchSlot g_scratch[kMaxDevices];
+
+}  // namespace lgx
+
+extern "C" int lgx_gather_noalloc(const lgx_gather_params *p, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+                                  lgx_image *out, lgx_stream_t stream) {
+  const int rc = check_gather_images(p, g, pyr, out);
+  if (rc != LGX_OK) return rc;
+  const GatherTableShape s = table_shape(p);
+  if (s.S_tab > LGX_MAX_STEPS) return LGX_EUNSUPPORTED;
+  if (lgx_gather_workspace_bytes(p) > (size_t)LGX_GATHER_SCRATCH_BYTES) return LGX_EUNSUPPORTED;
+  int dev = 0;
+  int hrc = hip_status(hipGetDevice(&dev));
+  if (hrc != LGX_OK) return hrc;
+  if (dev < 0 || dev >= kMaxDevices) return LGX_EUNSUPPORTED;
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  ScratchSlot &slot = g_scratch[dev];
+  if (!slot.ptr) {
+    void *ptr = nullptr;
+    hrc = hip_status(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (hrc != LGX_OK) return hrc;
+    if (!ptr) return LGX_EINVAL;
+    slot.ptr = ptr;
+  }
+  const TableKey key = table_key(p);
+  if (!slot.loaded || !(slot.key == key)) {
+    std::vector<unsigned char> t;
+    const bool ok = p->precision == LGX_PREC_F64 ? fill_tables<double>(p, s, t) : fill_tables<float>(p, s, t);
+    if (!ok) return LGX_EUNSUPPORTED;
+    // earlier gathers on this stream read the old tables: let them finish,
+    // then upload on the stream and wait, so the host buffer can go
+    slot.loaded = false;
+    hrc = hip_status(hipStreamSynchronize(st));
+    if (hrc == LGX_OK) hrc = hip_status(hipMemcpyAsync(slot.ptr, t.data(), t.size(), hipMemcpyHostToDevice, st));
+    if (hrc == LGX_OK) hrc = hip_status(hipStreamSynchronize(st));
+    if (hrc != LGX_OK) return hrc;
+    slot.key = key;
+    slot.loaded = true;
+  }
+  if (p->precision == LGX_PREC_F64) return gather_impl<double, double>(p, slot.ptr, g, pyr, out, st);
+  if (p->precision == LGX_PREC_F32_ACC64) return gather_impl<float, double>(p, slot.ptr, g, pyr, out, st);
+  return gather_impl<float, float>(p, slot.ptr, g, pyr, out, st);
+}

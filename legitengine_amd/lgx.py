@@ -93,6 +93,8 @@ SIGNATURES = {
     "lgx_gather": (C.c_int, [C.POINTER(GatherParams), C.c_void_p, C.POINTER(GBuffer), C.POINTER(Pyramid),
                              C.POINTER(Image), C.c_void_p]),
     "lgx_gather_tiles": (C.c_int, []),
+    "lgx_gather_noalloc": (C.c_int, [C.POINTER(GatherParams), C.POINTER(GBuffer), C.POINTER(Pyramid),
+                                     C.POINTER(Image), C.c_void_p]),
     "lgx_mip_build": (C.c_int, [C.POINTER(Image), C.c_int32, C.c_int32, C.c_void_p]),
     "lgx_box_blur": (C.c_int, [C.POINTER(Image), C.POINTER(Image), C.POINTER(C.c_int32), C.c_int32, C.c_void_p]),
     "lgx_deinterleave": (C.c_int, [C.POINTER(Image), C.POINTER(Image), C.c_int32, C.c_int32, C.c_void_p]),

@@ -70,6 +70,9 @@ def parse():
                     help="time the denoised path only (no error sweep against the FP64 target)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch a 256 MiB device copy 3x and the 1080p gather 3x (for rocprofv3 --pmc)")
+    ap.add_argument("--unet-probe", action="store_true",
+                    help="only run the 1080p U-Net denoiser pass 3x and 3 training steps (batch 16 of 256x256) "
+                         "with autotuned tiles (for rocprofv3 --pmc)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_1080p_plain_traffic.md"))
     ap.add_argument("--counters-file", default=os.path.join(ROOT, "profiles", "gather_1080p_plain_counters.md"))
     ap.add_argument("--quality-seeds", default=",".join(str(s) for s in QUALITY_SEEDS))
@@ -94,7 +97,10 @@ def main():
     view, proj = lgx.scene_matrices(scene, W, H)
     g = dev.GBufferDev.alloc(W, H, "mixed").render(scene)
     if args.traffic_probe:
-        traffic_probe(g, view, proj, sampler)
+        traffic_probe(g, view, proj, sampler, args.gather_tile)
+        return
+    if args.unet_probe:
+        unet_probe(args, g)
         return
     # The drop-in path: SsvgiRenderer passes declared on the C++ RenderGraph,
     # timed per pass with hipEvents on the frame's stream; the gather's launch
@@ -194,16 +200,20 @@ def main():
 COPY_BYTES = 256 << 20
 
 
-def traffic_probe(g, view, proj, sampler):
-    """Kernels for the HBM-traffic counters: a plain device copy of known size
-    (the calibration) and the bench's gather launch, three times each."""
+def traffic_probe(g, view, proj, sampler, tile=-1):
+    """Kernels for the counters: a plain device copy of known size (the
+    traffic calibration) and the bench's gather launch, three times each, in
+    the launch shape the bench times (`tile`, or the autotuner's pick; the
+    autotuner's own launches precede the copy)."""
     import torch
     from legitengine_amd import device as dev
     from legitengine_amd import lgx
+    pyr = dev.PyramidDev(g, LEVELS, sampler).build()
+    if tile < 0:
+        tile, _ = dev.autotune_gather(g, pyr, view, proj, DIRS, STEPS, sampler)
     x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
-    pyr = dev.PyramidDev(g, LEVELS, sampler).build()
-    ga = dev.Gather(W, H, DIRS, STEPS, sampler)
+    ga = dev.Gather(W, H, DIRS, STEPS, sampler, tile=tile)
     out = dev.DImage(W, H, lgx.RGBA32F)
     torch.cuda.synchronize()
     for _ in range(3):
@@ -212,8 +222,42 @@ def traffic_probe(g, view, proj, sampler):
     for _ in range(3):
         ga(view, proj, g, pyr, out=out)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "gather_launches": 3}),
-          flush=True)
+    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "gather_launches": 3,
+                      "tile": tile}), flush=True)
+
+
+def unet_probe(args, g):
+    """Kernels for the U-Net counters: a 256 MiB device copy 3x (the traffic
+    calibration), the 1080p denoiser pass (trained weights, autotuned tiles)
+    3x, then 3 training steps on batch 16 of 256x256 random records."""
+    import torch
+    from legitengine_amd import device as dev
+    from legitengine_amd import denoiser as dn
+    from legitengine_amd import lgx
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    net = dev.UNet(1, H, W)
+    net.load(args.weights or dn.WEIGHTS)
+    net.autotune(1, H, W)
+    noisy = dev.DImage(W, H, lgx.RGBA32F)
+    noisy.t.uniform_(0, 1)
+    out = dev.DImage(W, H, lgx.RGBA32F)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        net.denoise(g, noisy, out=out)
+    torch.cuda.synchronize()
+    tnet = dev.UNet(16, dn.CROP, dn.CROP, seed=0)
+    tnet.autotune(16, dn.CROP, dn.CROP, backward=True)
+    xb = torch.rand((16, dn.CROP, dn.CROP, 12), device="cuda")
+    tb = torch.rand((16, dn.CROP, dn.CROP, 4), device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        tnet.train_step(xb, tb, 1e-3)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": COPY_BYTES, "denoise_launches": 3, "train_steps": 3}), flush=True)
 
 
 def read_traffic(path):
@@ -230,8 +274,8 @@ def read_traffic(path):
 
 def read_counters(path):
     """Measured utilisation of the gather from the committed counter summary
-    (lines 'key: value' for valu_busy, valu_utilization, l2_hit,
-    mem_unit_busy, ...), or {}."""
+    (lines 'counter_<key>: value': VALU issue and lane utilisation, waves
+    per SIMD, L1 / L2 hit rates, texture-address busy, ...), or {}."""
     out = {}
     try:
         for line in open(path):
@@ -244,7 +288,12 @@ def read_counters(path):
                     pass
     except OSError:
         return {}
-    return {"measured": out} if out else {}
+    if not out:
+        return {}
+    res = {"measured": out}
+    if "valu_issue_busy" in out:  # the unit that binds in practice: VALU issue slots used
+        res["valu_frac"] = out["valu_issue_busy"]
+    return res
 
 
 def time_renderer(r, g, view, proj, steps, warmup):
@@ -462,6 +511,33 @@ def batch_leg(args, sampler):
     counts = {f"{w}x{h}": sizes.count((w, h)) for (w, h) in BATCH_SIZES}
     tile, wg = b.shape()
     b.close()
+    # the same views one at a time through the single-view path (pyramids,
+    # lgx_gather with the batch's launch shape, lgx_unet_denoise), smallest
+    # first, eager on one stream: what the batch is measured against
+    seq = []
+    for i in sorted(range(len(views)), key=lambda i: sizes[i][0] * sizes[i][1]):
+        (g, view, proj), (w, h) = views[i], sizes[i]
+        ga = dev.Gather(w, h, DEN_DIRS, dn.NOISY_STEPS, sampler, grid=dn.NOISY_GRID, tile=tile,
+                        footprint_dirs=dn.NOISY_FOOTPRINT)
+        seq.append((g, view, proj, dev.PyramidDev(g, LEVELS, sampler), ga, dev.DImage(w, h, lgx.RGBA32F),
+                    dev.DImage(w, h, lgx.RGBA32F)))
+
+    def sequential():
+        for g, view, proj, pyr, ga, noisy, den in seq:
+            pyr.build(stream)
+            ga(view, proj, g, pyr, out=noisy, stream=stream)
+            net.denoise(g, noisy, out=den, stream=stream)
+
+    sequential()
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        sequential()
+    e1.record(stream)
+    e1.synchronize()
+    seq_ms = e0.elapsed_time(e1) / args.steps
+    del seq
     return {
         "workload": f"{len(views)} views (seeds {BATCH_SEEDS[0]}-{BATCH_SEEDS[-1]}, own scenes), resolutions drawn "
                     "from {512x512, 1280x720, 1920x1080, 2560x1440}: mips, Hi-Z and pre-blur per view, one "
@@ -471,6 +547,10 @@ def batch_leg(args, sampler):
         "megapixels": round(pixels / 1e6, 3),
         "batch_ms": round(batch_ms, 4),
         "mpixel_per_s": round(pixels / 1e6 / (batch_ms * 1e-3), 1),
+        "sequential_ms": round(seq_ms, 4),
+        "sequential_mpixel_per_s": round(pixels / 1e6 / (seq_ms * 1e-3), 1),
+        "sequential_note": "the same views one at a time (pyramids, lgx_gather, lgx_unet_denoise), smallest first, "
+                           "eager on one stream",
         "stamped_batch_ms": round(float(np.median(stamped)), 4),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "min": round(float(min(lat)), 4)},

@@ -360,11 +360,12 @@ int64_t lgx_gather_batch_blocks(const lgx_gather_batch *b);
 int lgx_gather_batch_launch(const lgx_gather_batch *b, int32_t workgroups, lgx_stream_t stream);
 void lgx_gather_batch_destroy(lgx_gather_batch *b);
 
-/* lgx_batch: the denoised path for n views (own G-buffers and cameras):
- * per view the mips, Hi-Z and pre-blur, then the 4-direction gather of all
- * views in one lgx_gather_batch launch, then per view (smallest first) the
- * U-Net denoiser, and a device timestamp as each view completes.  The whole
- * batch can be captured into ONE hipGraph and replayed. */
+/* lgx_batch: the denoised path for n views (own G-buffers and cameras).
+ * The views, in ascending pixel count, form `chunks` chunks; per chunk the
+ * mips, Hi-Z and pre-blur of its views and ONE persistent lgx_gather_batch
+ * launch over them, then the U-Net of each of its views, smallest first,
+ * and a device timestamp as each view completes, all on the caller's stream.  The whole batch can be
+ * captured into ONE hipGraph and replayed. */
 typedef struct lgx_batch lgx_batch;
 typedef struct {
   int32_t width, height;
@@ -376,6 +377,8 @@ typedef struct {
   int32_t tile;       /* gather launch shape */
   int32_t workgroups; /* persistent gather workgroups, 0 = one per block */
   int32_t grid_y;     /* interleave pattern height (lgx_gather_params), <= 0 = grid */
+  int32_t chunks;     /* views, in ascending pixel count, are gathered in this many persistent launches
+                         (<= 0: 4); each chunk's U-Nets run before the next chunk's gather */
 } lgx_batch_config;
 /* net: created with max n >= 1 and max h x w covering every view; NULL =
  * no denoiser (the views' outputs are the noisy gathers). */

@@ -1,12 +1,16 @@
 // lgx_batch: the denoised GI path for many views of uneven resolution,
-// issued as one stream of work that can be captured into one hipGraph.
+// issued on one stream and capturable into one hipGraph.
 //
-// Per submission: the mips / Hi-Z / pre-blur of every view (small launches),
-// the 4-direction gather of ALL views in one persistent launch
-// (lgx_gather_batch), then the U-Net per view, smallest view first, so the
-// small views complete early; a one-thread kernel stamps the device wall
-// clock at the start and after each view, which gives per-view latencies
-// from inside a graph replay without host round trips.
+// The views, in ascending pixel count, form a few chunks.  Per chunk the
+// stream builds the views' mips / Hi-Z / pre-blur (small launches), gathers
+// all of the chunk's views in one persistent launch (lgx_gather_batch) and
+// then runs the U-Net of each of the chunk's views, smallest first.  Small
+// views therefore complete after a fraction of the batch.  (U-Nets on a
+// second stream, overlapping the next chunk's gather, waited for the
+// persistent gather's workgroups to drain from the CUs and finished near the
+// end of the batch.)  A one-thread kernel stamps the device wall clock at the
+// start and after each view, which gives per-view latencies from inside a
+// graph replay without host round trips.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,10 +53,11 @@ using namespace lgx;
 struct lgx_batch {
   lgx_batch_config c{};
   std::vector<BatchView> v;
-  std::vector<int> order;  // denoiser order: ascending pixel count
+  std::vector<int> order;                 // ascending pixel count
+  std::vector<std::vector<int>> chunks;   // consecutive runs of `order`
   std::vector<void *> blocks;
   std::map<std::pair<int, int>, void *> ws;  // gather tables per extent
-  lgx_gather_batch *gb = nullptr;
+  std::vector<lgx_gather_batch *> gbs;    // one persistent gather per chunk
   lgx_unet *net = nullptr;
   unsigned long long *ts = nullptr;  // device [n + 1]
   hipGraph_t graph = nullptr;
@@ -63,7 +68,7 @@ struct lgx_batch {
   ~lgx_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (gb) lgx_gather_batch_destroy(gb);
+    for (lgx_gather_batch *gb : gbs) lgx_gather_batch_destroy(gb);
     for (void *p : blocks) (void)hipFree(p);
   }
   void *alloc(size_t bytes) {
@@ -102,25 +107,36 @@ struct lgx_batch {
     hipc(hipGetLastError(), "stamp_kernel");
   }
 
-  void enqueue(hipStream_t s) {
+  void pyramids(BatchView &bv, hipStream_t s) {
     const lgx_stream_t ls = (lgx_stream_t)s;
-    stamp(0, s);
     const int L = c.levels;
-    std::vector<int32_t> radius((size_t)std::max(L - 1, 1), 2);
-    for (BatchView &bv : v) {
-      if (L < 2) continue;
-      Check(lgx_mip_build(bv.rad.data(), L, LGX_MIP_AVG, ls), "lgx_mip_build radiance");
-      Check(lgx_mip_build(bv.dep.data(), L, c.sampler == LGX_SAMPLER_HIZ_POINT ? LGX_MIP_MAX : LGX_MIP_AVG, ls),
-            "lgx_mip_build depth");
-      Check(lgx_box_blur(bv.rad.data() + 1, bv.rad_b.data() + 1, radius.data(), L - 1, ls), "lgx_box_blur radiance");
-      if (c.sampler == LGX_SAMPLER_TRILINEAR)
-        Check(lgx_box_blur(bv.dep.data() + 1, bv.dep_b.data() + 1, radius.data(), L - 1, ls), "lgx_box_blur depth");
+    if (L < 2) return;
+    std::vector<int32_t> radius((size_t)(L - 1), 2);
+    Check(lgx_mip_build(bv.rad.data(), L, LGX_MIP_AVG, ls), "lgx_mip_build radiance");
+    Check(lgx_mip_build(bv.dep.data(), L, c.sampler == LGX_SAMPLER_HIZ_POINT ? LGX_MIP_MAX : LGX_MIP_AVG, ls),
+          "lgx_mip_build depth");
+    Check(lgx_box_blur(bv.rad.data() + 1, bv.rad_b.data() + 1, radius.data(), L - 1, ls), "lgx_box_blur radiance");
+    if (c.sampler == LGX_SAMPLER_TRILINEAR)
+      Check(lgx_box_blur(bv.dep.data() + 1, bv.dep_b.data() + 1, radius.data(), L - 1, ls), "lgx_box_blur depth");
+  }
+
+  void enqueue(hipStream_t s) {
+    stamp(0, s);
+    for (size_t k = 0; k < chunks.size(); k++) {
+      for (int i : chunks[k]) pyramids(v[i], s);
+      Check(lgx_gather_batch_launch(gbs[k], c.workgroups, (lgx_stream_t)s), "lgx_gather_batch_launch");
+      for (int i : chunks[k]) {
+        if (net)
+          Check(lgx_unet_denoise(net, &v[i].g, &v[i].noisy, &v[i].denoised, (lgx_stream_t)s), "lgx_unet_denoise");
+        stamp(1 + i, s);
+      }
     }
-    Check(lgx_gather_batch_launch(gb, c.workgroups, ls), "lgx_gather_batch_launch");
-    for (int i : order) {
-      if (net) Check(lgx_unet_denoise(net, &v[i].g, &v[i].noisy, &v[i].denoised, ls), "lgx_unet_denoise");
-      stamp(1 + i, s);
-    }
+  }
+
+  // Persistent gathers of every chunk, back to back (the autotuner's unit).
+  void launch_gathers(hipStream_t s) {
+    for (lgx_gather_batch *gb : gbs)
+      Check(lgx_gather_batch_launch(gb, c.workgroups, (lgx_stream_t)s), "lgx_gather_batch_launch");
   }
 };
 
@@ -194,25 +210,35 @@ extern "C" int lgx_batch_create(const lgx_batch_config *c, const lgx_batch_view 
         b->ws[key] = w;
       }
     }
-    std::vector<lgx_gather_params> ps;
-    std::vector<const void *> wss;
-    std::vector<lgx_gbuffer> gs;
-    std::vector<lgx_pyramid> pyrs;
-    std::vector<lgx_image> outs;
-    for (BatchView &bv : b->v) {
-      ps.push_back(b->params(bv));
-      wss.push_back(b->ws[std::make_pair(bv.w, bv.h)]);
-      gs.push_back(bv.g);
-      pyrs.push_back(bv.pyr);
-      outs.push_back(bv.noisy);
-    }
-    Check(lgx_gather_batch_create(ps.data(), wss.data(), gs.data(), pyrs.data(), outs.data(), n, &b->gb),
-          "lgx_gather_batch_create");
     b->order.resize((size_t)n);
     for (int i = 0; i < n; i++) b->order[i] = i;
     std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
       return (long)b->v[x].w * b->v[x].h < (long)b->v[y].w * b->v[y].h;
     });
+    // chunks: consecutive runs of the ascending order, equal view counts
+    const int K = std::min(n, c->chunks > 0 ? (int)c->chunks : 4);
+    for (int k = 0; k < K; k++) {
+      std::vector<int> ch(b->order.begin() + (size_t)k * n / K, b->order.begin() + (size_t)(k + 1) * n / K);
+      std::vector<lgx_gather_params> ps;
+      std::vector<const void *> wss;
+      std::vector<lgx_gbuffer> gs;
+      std::vector<lgx_pyramid> pyrs;
+      std::vector<lgx_image> outs;
+      for (int i : ch) {
+        BatchView &bv = b->v[i];
+        ps.push_back(b->params(bv));
+        wss.push_back(b->ws[std::make_pair(bv.w, bv.h)]);
+        gs.push_back(bv.g);
+        pyrs.push_back(bv.pyr);
+        outs.push_back(bv.noisy);
+      }
+      lgx_gather_batch *gb = nullptr;
+      Check(lgx_gather_batch_create(ps.data(), wss.data(), gs.data(), pyrs.data(), outs.data(), (int32_t)ch.size(),
+                                    &gb),
+            "lgx_gather_batch_create");
+      b->gbs.push_back(gb);
+      b->chunks.push_back(ch);
+    }
     b->ts = (unsigned long long *)b->alloc(sizeof(unsigned long long) * ((size_t)n + 1));
     int dev = 0, khz = 0;
     hipc(hipGetDevice(&dev), "hipGetDevice");
@@ -280,16 +306,18 @@ extern "C" int lgx_batch_autotune(lgx_batch *b, lgx_stream_t stream, char *buf, 
     double best = 1e30;
     int best_tile = b->c.tile, best_wg = b->c.workgroups;
     const int wg_mult[] = {0, 4, 8, 16};
+    const int saved_wg = b->c.workgroups;
     try {
       for (int t = 0; t < lgx_gather_tiles(); t++) {
-        Check(lgx_gather_batch_set_tile(b->gb, t), "lgx_gather_batch_set_tile");
+        for (lgx_gather_batch *gb : b->gbs) Check(lgx_gather_batch_set_tile(gb, t), "lgx_gather_batch_set_tile");
         for (int m : wg_mult) {
           const int wg = m * cus;
-          Check(lgx_gather_batch_launch(b->gb, wg, (lgx_stream_t)s), "lgx_gather_batch_launch");  // warm-up
+          b->c.workgroups = wg;
+          b->launch_gathers(s);  // warm-up
           std::vector<float> ms;
           for (int r = 0; r < 3; r++) {
             hipc(hipEventRecord(e0, s), "hipEventRecord");
-            Check(lgx_gather_batch_launch(b->gb, wg, (lgx_stream_t)s), "lgx_gather_batch_launch");
+            b->launch_gathers(s);
             hipc(hipEventRecord(e1, s), "hipEventRecord");
             hipc(hipEventSynchronize(e1), "hipEventSynchronize");
             float x = 0;
@@ -309,6 +337,7 @@ extern "C" int lgx_batch_autotune(lgx_batch *b, lgx_stream_t stream, char *buf, 
         }
       }
     } catch (...) {
+      b->c.workgroups = saved_wg;
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       throw;
@@ -316,7 +345,7 @@ extern "C" int lgx_batch_autotune(lgx_batch *b, lgx_stream_t stream, char *buf, 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     js += "]";
-    Check(lgx_gather_batch_set_tile(b->gb, best_tile), "lgx_gather_batch_set_tile");
+    for (lgx_gather_batch *gb : b->gbs) Check(lgx_gather_batch_set_tile(gb, best_tile), "lgx_gather_batch_set_tile");
     b->c.tile = best_tile;
     b->c.workgroups = best_wg;
     if (buf && cap) {

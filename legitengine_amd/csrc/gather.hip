@@ -21,6 +21,7 @@
 //    texels as the scalar restatement.
 //  * XCD-aware block order: the 16 slices of a tile run back to back on one
 //    XCD, neighbouring tiles stay on that XCD's L2.
+#include <limits.h>
 #include <math.h>
 
 #include <mutex>
@@ -84,6 +85,56 @@ template <class T> LGX_D T div_extent(T q, T b, T rb) {
   const T e = fma(-y, b, q);
   return fma(e, rb, y);
 }
+
+// atan2(y, x) for finite arguments (atan2(0, 0) = 0), float: octant
+// reduction to t = min(|x|, |y|) / max(|x|, |y|) in [0, 1] and
+// atan(t) = t P(t^2) with a degree-7 polynomial fitted to atan(t) / t on
+// [0, 1] by iteratively reweighted least squares (max error 4e-8; 1.2e-7
+// evaluated in float).  The hit branch of the march calls it on every horizon
+// rise, where the library routine's handling of infinities and NaNs (which
+// cannot occur there) costs more than the polynomial.
+template <class T> LGX_D T atan2_finite(T y, T x);
+template <> LGX_D double atan2_finite<double>(double y, double x) { return atan2(y, x); }
+template <> LGX_D float atan2_finite<float>(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+  const float s = t * t;
+  float p = -0.004054928198456764f;
+  p = fmaf(p, s, 0.021864214912056923f);
+  p = fmaf(p, s, -0.05591406673192978f);
+  p = fmaf(p, s, 0.0964231863617897f);
+  p = fmaf(p, s, -0.1390867531299591f);
+  p = fmaf(p, s, 0.19946573674678802f);
+  p = fmaf(p, s, -0.33329862356185913f);
+  p = fmaf(p, s, 0.9999993443489075f);
+  float r = t * p;
+  r = ay > ax ? 1.5707963705062866f - r : r;
+  r = x < 0.f ? 3.1415927410125732f - r : r;
+  return copysignf(r, y);
+}
+
+// Radiance texel through a buffer resource (32-bit byte offset against a
+// scalar base), widened to float.
+template <int FMT> struct RadBuf;
+template <> struct RadBuf<LGX_RGBA16F> {
+  static constexpr int kBytes = 8;
+  static LGX_D float4 load(__amdgpu_buffer_rsrc_t r, uint32_t ti) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(ti * 8u), 0, 0);
+    return make_float4(h2f((uint16_t)(v[0] & 0xffffu)), h2f((uint16_t)(v[0] >> 16)), h2f((uint16_t)(v[1] & 0xffffu)),
+                       h2f((uint16_t)(v[1] >> 16)));
+  }
+};
+template <> struct RadBuf<LGX_RGBA32F> {
+  static constexpr int kBytes = 16;
+  static LGX_D float4 load(__amdgpu_buffer_rsrc_t r, uint32_t ti) {
+    // the whole vector is reinterpreted at once: a bit cast of one element
+    // of the loaded vector (v[i]) compiles to a read of element 0 here
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(ti * 16u), 0, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+};
 
 // Monotone stand-in for atan2(y, x) on (-pi, pi]: values in [-2, 2].  Only
 // its order matters; near-ties decide steps whose contribution h(theta,
@@ -236,70 +287,91 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
     const T trc = rc * dot3(tangent, rayc);
     int j = 0;
     if (SAMPLER == LGX_SAMPLER_HIZ_POINT) {
-      // Level-major march: the Hi-Z level floor(lod) never decreases along a
-      // ray, so each level's constants stay in scalar registers for its run
-      // of steps.
+      // Level-major march.  The step index j is the same on every active
+      // lane (all start at 0 and advance together; lanes only leave), so
+      // it is read as a scalar; the Hi-Z level floor(lod) never decreases
+      // along a ray, so the level changes at table-given steps and its base,
+      // pitch and extent stay in scalar registers for its run of steps.
       const int *lstart = a.level_start + (size_t)k * (LGX_MAX_LEVELS + 1);
-      for (int l = 0; l < a.n_levels; l++) {
-        const int jend = (l == a.n_levels - 1) ? iters : min(iters, lstart[l + 1]);
-        if (j >= jend) continue;
-        const float *dl = a.pd[l];
-        const void *rl = a.pr[l];
-        const int dpitch = a.pd_pitch[l], rpitch = a.pr_pitch[l];
-        // the level's depth as a buffer resource: the per-step fetch takes a
-        // 32-bit byte offset against a scalar base (no 64-bit address math)
-        const __amdgpu_buffer_rsrc_t drs =
-            __builtin_amdgcn_make_buffer_rsrc((void *)dl, (short)0, a.lh[l] * dpitch * 4, kBufferConfig);
-        const T flw = T(a.lw[l]), flh = T(a.lh[l]);
-        const int lw1 = a.lw[l] - 1, lh1 = a.lh[l] - 1;
-        for (; j < jend; j++) {
-          // every active lane is at the same step (all start a level at its
-          // first step and advance together), so the table read is uniform
-          const int ju = __builtin_amdgcn_readfirstlane(j);
-          const T off = steps[2 * ju];
-          const T qx = cx + dx * off, qy = cy + dy * off;  // :218
-          const T sqx = div_extent(qx, a.W, a.rW), sqy = div_extent(qy, a.H, a.rH);  // :219
-          if (!(sqx > T(0) && sqx < T(1) && sqy > T(0) && sqy < T(1))) goto dir_done;
-          // on screen: sq > 0, so floor(sq * extent) >= 0; the product can round
-          // up to the extent itself, hence the upper clamp (in integers)
-          const int ix = min((int)floor(sqx * flw), lw1);
-          const int iy = min((int)floor(sqy * flh), lh1);
-          const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
-          const T r = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0)));
-          // :241-252 sample delta projected on (eye, tangent)
-          const T len2 = fma_(off, fma_(off, vv, uv2), uu);
-          const T ri = r * fast_rsqrt(len2);
-          const T hyv = ri * fma_(off, tv, tu) - trc;
-          const T hxv = ri * fma_(off, ev, eu) - erc;
-          const T pa = pseudo_angle(hyv, hxv);
-          if (pa < pa_max) {  // :254
-            if (COUNT) hit_total++;
-            const T th = atan2(hyv, hxv);
-            const T iq = fast_rcp(hxv * hxv + hyv * hyv);
-            const T c2 = (hxv * hxv - hyv * hyv) * iq, s2 = (T(2) * hxv * hyv) * iq;
-            T side = T(1);
-            {
-              const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
-              side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
-              side *= min(max(sqx * inv_width, T(0)), T(1));
-              side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
-              side *= min(max(sqy * inv_width, T(0)), T(1));
-            }
-            const T hc =
-                (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
-            const float4 v = Texel<RFMT>::load(rl, (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix);  // :256
-            L0 += A(T(v.x) * hc);  // :261
-            L1 += A(T(v.y) * hc);
-            L2 += A(T(v.z) * hc);
-            L0 -= A(ambient * hc);  // :262
-            L1 -= A(ambient * hc);
-            L2 -= A(ambient * hc);
-            max_h = th;  // :263
-            pa_max = pa;
-            c2max = c2;
-            s2max = s2;
-          }
+      int l = 0;
+      int lnext = a.n_levels > 1 ? lstart[1] : INT_MAX;  // first step of the next level
+      // the level's depth and radiance as buffer resources: per-step fetches
+      // take a 32-bit byte offset against a scalar base
+      auto depth_rsrc = [&](int lv) {
+        return __builtin_amdgcn_make_buffer_rsrc((void *)a.pd[lv], (short)0, a.lh[lv] * a.pd_pitch[lv] * 4,
+                                                 kBufferConfig);
+      };
+      auto rad_rsrc = [&](int lv) {
+        return __builtin_amdgcn_make_buffer_rsrc((void *)a.pr[lv], (short)0,
+                                                 a.lh[lv] * a.pr_pitch[lv] * RadBuf<RFMT>::kBytes, kBufferConfig);
+      };
+      __amdgpu_buffer_rsrc_t drs = depth_rsrc(0), rrs = rad_rsrc(0);
+      T flw = T(a.lw[0]), flh = T(a.lh[0]);
+      int lw1 = a.lw[0] - 1, lh1 = a.lh[0] - 1, dpitch = a.pd_pitch[0], rpitch = a.pr_pitch[0];
+      T off = steps[0];
+      for (;; j++) {
+        if (j >= iters) break;
+        const int ju = __builtin_amdgcn_readfirstlane(j);
+        // the next step's offset: a scalar load in flight during this step
+        const T off_next = steps[2 * min(ju + 1, a.S_tab - 1)];
+        while (ju >= lnext) {  // uniform: step ju lies on a later level
+          l++;
+          lnext = l + 1 < a.n_levels ? lstart[l + 1] : INT_MAX;
+          drs = depth_rsrc(l);
+          rrs = rad_rsrc(l);
+          dpitch = a.pd_pitch[l];
+          rpitch = a.pr_pitch[l];
+          flw = T(a.lw[l]);
+          flh = T(a.lh[l]);
+          lw1 = a.lw[l] - 1;
+          lh1 = a.lh[l] - 1;
         }
+        const T qx = cx + dx * off, qy = cy + dy * off;  // :218
+        // :219-228 on screen: q / extent in (0, 1) is q in (0, extent), as
+        // the quotient is correctly rounded (it reaches 1 only from q = extent)
+        // (bitwise &: one mask, not a chain of divergent branches)
+        if (!((qx > T(0)) & (qx < a.W) & (qy > T(0)) & (qy < a.H))) break;
+        const T sqx = div_extent(qx, a.W, a.rW), sqy = div_extent(qy, a.H, a.rH);  // :219
+        // sq > 0, so sq * extent >= 0 and truncation is floor; the product can
+        // round up to the extent itself, hence the upper clamp (in integers)
+        const int ix = min((int)(sqx * flw), lw1);
+        const int iy = min((int)(sqy * flh), lh1);
+        const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
+        const T r = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0)));
+        // :241-252 sample delta projected on (eye, tangent)
+        const T len2 = fma_(off, fma_(off, vv, uv2), uu);
+        const T ri = r * fast_rsqrt(len2);
+        const T hyv = ri * fma_(off, tv, tu) - trc;
+        const T hxv = ri * fma_(off, ev, eu) - erc;
+        const T pa = pseudo_angle(hyv, hxv);
+        if (pa < pa_max) {  // :254
+          if (COUNT) hit_total++;
+          const T th = atan2_finite(hyv, hxv);
+          const T iq = fast_rcp(hxv * hxv + hyv * hyv);
+          const T c2 = (hxv * hxv - hyv * hyv) * iq, s2 = (T(2) * hxv * hyv) * iq;
+          T side = T(1);
+          {
+            const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
+            side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
+            side *= min(max(sqx * inv_width, T(0)), T(1));
+            side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
+            side *= min(max(sqy * inv_width, T(0)), T(1));
+          }
+          const T hc =
+              (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
+          const float4 v = RadBuf<RFMT>::load(rrs, (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix);  // :256
+          L0 += A(T(v.x) * hc);  // :261
+          L1 += A(T(v.y) * hc);
+          L2 += A(T(v.z) * hc);
+          L0 -= A(ambient * hc);  // :262
+          L1 -= A(ambient * hc);
+          L2 -= A(ambient * hc);
+          max_h = th;  // :263
+          pa_max = pa;
+          c2max = c2;
+          s2max = s2;
+        }
+        off = off_next;
       }
     } else {
       for (; j < iters; j++) {
@@ -361,7 +433,6 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
         }
       }
     }
-  dir_done:
     if (COUNT) step_total += j;
     sum0 += A(2) * L0 / A(a.D);  // :268
     sum1 += A(2) * L1 / A(a.D);
@@ -400,11 +471,12 @@ __global__ __launch_bounds__(256, OCC) void gather_kernel(const GatherArgs<T> a)
 // one list (view-major, first[v] = index of view v's first block, first[V] =
 // total).  The list is cut into 8 contiguous chunks, one per XCD (workgroup
 // w runs on XCD w % 8, gridDim.x is a multiple of 8), so neighbouring tiles
-// and the 16 slices of a tile share one L2.  Within its chunk a workgroup
-// takes the next block from the XCD's counter (zeroed before the launch):
-// blocks differ in cost (early exits, view content), and a fixed stride
-// would leave workgroups idle at the tail.  Every workgroup ends when its
-// counter passes the chunk end.
+// and the 16 slices of a tile share one L2.  Within a chunk a workgroup takes
+// the next block from the chunk's counter (zeroed before the launch): blocks
+// differ in cost (early exits, view content), and a fixed stride would leave
+// workgroups idle at the tail.  A workgroup whose own chunk is exhausted
+// takes blocks from the other XCDs' chunks in turn (the counters only grow,
+// so every block is taken once), and ends when all eight are exhausted.
 template <class T, class A, int SAMPLER, int RFMT, int NFMT, int OCC>
 __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs<T> *__restrict__ views,
                                                            const uint32_t *__restrict__ first, int n_views,
@@ -414,16 +486,19 @@ __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs
   const uint32_t total = first[n_views];
   const uint32_t xcd = blockIdx.x % X;
   const uint32_t chunk = (total + X - 1) / X;
-  const uint32_t c0 = xcd * chunk, c1 = min(total, c0 + chunk);
-  int v = 0;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = c0 + atomicAdd(&next[xcd], 1u);
-    __syncthreads();
-    const uint32_t it = s_item;
-    __syncthreads();  // every wave has read s_item before lane 0 overwrites it
-    if (it >= c1) break;
-    while (v + 1 < n_views && it >= first[v + 1]) v++;  // a workgroup's items only grow
-    gather_block<T, A, SAMPLER, RFMT, NFMT, false>(views[v], it - first[v]);
+  for (uint32_t k = 0; k < X; k++) {
+    const uint32_t y = (xcd + k) % X;  // own chunk first, then the others
+    const uint32_t c0 = min(total, y * chunk), c1 = min(total, c0 + chunk);
+    int v = 0;
+    for (;;) {
+      if (threadIdx.x == 0) s_item = c0 + atomicAdd(&next[y], 1u);
+      __syncthreads();
+      const uint32_t it = s_item;
+      __syncthreads();  // every wave has read s_item before lane 0 overwrites it
+      if (it >= c1) break;
+      while (v + 1 < n_views && it >= first[v + 1]) v++;  // items of one chunk only grow
+      gather_block<T, A, SAMPLER, RFMT, NFMT, false>(views[v], it - first[v]);
+    }
   }
 }
 

@@ -31,6 +31,19 @@ struct ConvArgs {
   const float4 *head_noisy;      // residual, row pitch head_noisy_pitch
   float4 *head_out;              // row pitch head_out_pitch; null = no head
   int head_noisy_pitch, head_out_pitch;
+  // First convolution at inference (cout = 32, an LDS tile, n = 1): the
+  // input records are packed from the noisy image and the G-buffer while
+  // the tile is staged, channels (noisy rgb, depth, normal xy, albedo rgb,
+  // zeros), as pack_gbuffer32 would write them; a and b are unused.
+  int gin_fmt;                   // 0: off; 1: RG16F normal + RGBA8 albedo; 2: RG32F + RGBA32F
+  const float4 *gin_noisy;
+  const float *gin_depth;
+  const void *gin_normal, *gin_albedo;
+  int gin_noisy_pitch, gin_depth_pitch, gin_normal_pitch, gin_albedo_pitch;
+  // Also write the 2x2 average pool of the output (NHWC, h/2 x w/2, cout
+  // channels), summed as avgpool_kernel sums it; null = off.  Needs an LDS
+  // tile whose waves hold an even number of rows (conv3x3_tile_pool).
+  bf16 *pool_out;
 };
 
 struct WgradArgs {
@@ -52,6 +65,7 @@ struct WgradArgs {
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
 int conv3x3_tiles(int cout);  // register tiles offered for this output channel count
 int conv3x3_tile_lds(int cout, int tile);  // 1 if that tile is an LDS tile covering all cout channels
+int conv3x3_tile_pool(int cout, int tile);  // 1 if that tile can write the 2x2 pooled output (pool_out)
 hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s);
 size_t conv3x3_wgrad_scratch_bytes();
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);

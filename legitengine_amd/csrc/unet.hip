@@ -167,7 +167,7 @@ class Net {
   }
 
   void conv(int l, const bf16 *a, int ca, int a_up, const bf16 *b, int cb, int n, int h, int w, bf16 *out_,
-            hipStream_t s) {
+            hipStream_t s, bf16 *pool_out = nullptr) {
     ConvArgs c{};
     c.a = a;
     c.ca = ca;
@@ -184,7 +184,20 @@ class Net {
     c.out0 = out_;
     c.split = c.cout;
     c.tile = fwd_tile[l];
+    c.pool_out = pool_out;
     check(conv3x3(c, s), "conv3x3");
+  }
+
+  // convolution l followed by the 2x2 average pool of its output into pool:
+  // in the convolution's epilogue when its tile offers that, else a pass of its own
+  void conv_pool(int l, const bf16 *a, int ca, int n, int h, int w, bf16 *out_, bf16 *pool, hipStream_t s) {
+    const int cout = kLayers[l].cout;
+    if (conv3x3_tile_pool(cout, fwd_tile[l])) {
+      conv(l, a, ca, 0, nullptr, 0, n, h, w, out_, s, pool);
+    } else {
+      conv(l, a, ca, 0, nullptr, 0, n, h, w, out_, s);
+      check(avgpool(out_, n, h, w, cout, pool, s), "avgpool");
+    }
   }
 
   // input gradient of layer l: dy (cout channels) -> [split | rest] with masks
@@ -232,9 +245,9 @@ class Net {
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
     switch (l) {
       case 0: conv(0, x32, 32, 0, nullptr, 0, n, h, w, e1a, s); break;
-      case 1: conv(1, e1a, 32, 0, nullptr, 0, n, h, w, e1, s); break;
+      case 1: conv_pool(1, e1a, 32, n, h, w, e1, p1, s); break;  // and p1 = pool(e1)
       case 2: conv(2, p1, 32, 0, nullptr, 0, n, h2, w2, e2a, s); break;
-      case 3: conv(3, e2a, 64, 0, nullptr, 0, n, h2, w2, e2, s); break;
+      case 3: conv_pool(3, e2a, 64, n, h2, w2, e2, p2, s); break;  // and p2 = pool(e2)
       case 4: conv(4, p2, 64, 0, nullptr, 0, n, h4, w4, b1, s); break;
       case 5: conv(5, b1, 128, 0, nullptr, 0, n, h4, w4, b2, s); break;
       case 6: conv(6, b2, 128, 1, e2, 64, n, h2, w2, d2a, s); break;  // [up(b2), e2]
@@ -260,16 +273,9 @@ class Net {
     }
   }
 
-  // the convolutions and pools from x32 to d1 (layers below `upto`)
-  void body(int n, int h, int w, hipStream_t s, int upto = 10) {
-    const int h2 = h / 2, w2 = w / 2;
-    fwd_layer(0, n, h, w, s);
-    fwd_layer(1, n, h, w, s);
-    check(avgpool(e1, n, h, w, 32, p1, s), "avgpool");
-    fwd_layer(2, n, h, w, s);
-    fwd_layer(3, n, h, w, s);
-    check(avgpool(e2, n, h2, w2, 64, p2, s), "avgpool");
-    for (int l = 4; l < upto; l++) fwd_layer(l, n, h, w, s);
+  // forward convolutions [from, upto) from x32 towards d1 (layers 1 and 3 also pool)
+  void body(int n, int h, int w, hipStream_t s, int upto = 10, int from = 0) {
+    for (int l = from; l < upto; l++) fwd_layer(l, n, h, w, s);
   }
 
   void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
@@ -283,10 +289,43 @@ class Net {
 
   // inference straight from a G-buffer and the noisy image (same values as
   // packing the records and calling forward)
+  // An LDS tile of a 32-channel convolution: the autotuned one for layer l, else the first offered.
+  int lds_tile32(int l) const {
+    if (conv3x3_tile_lds(32, fwd_tile[l])) return fwd_tile[l];
+    for (int t = 0; t < conv3x3_tiles(32); t++)
+      if (conv3x3_tile_lds(32, t)) return t;
+    return 0;
+  }
+
   void denoise(const lgx_image &noisy, const lgx_gbuffer &g, const lgx_image &out_, hipStream_t s) {
     const int h = noisy.h, w = noisy.w;
-    check(pack_gbuffer32(noisy, g, x32, s), "pack_gbuffer32");
-    body(1, h, w, s, 9);
+    // the first convolution packs its input records from the G-buffer and
+    // the noisy image while staging them (no 32-channel input tensor)
+    {
+      ConvArgs c{};
+      c.ca = 32;
+      c.n = 1;
+      c.h = h;
+      c.w = w;
+      c.wt = wf[0];
+      c.bias = params + b_off[0];
+      c.cout = 32;
+      c.relu = 1;
+      c.out0 = e1a;
+      c.split = 32;
+      c.tile = lds_tile32(0);
+      c.gin_fmt = g.normal.format == LGX_RG16F ? 1 : 2;
+      c.gin_noisy = (const float4 *)noisy.ptr;
+      c.gin_noisy_pitch = noisy.pitch;
+      c.gin_depth = (const float *)g.depth.ptr;
+      c.gin_depth_pitch = g.depth.pitch;
+      c.gin_normal = g.normal.ptr;
+      c.gin_normal_pitch = g.normal.pitch;
+      c.gin_albedo = g.albedo.ptr;
+      c.gin_albedo_pitch = g.albedo.pitch;
+      check(conv3x3(c, s), "conv3x3 from the G-buffer");
+    }
+    body(1, h, w, s, 9, 1);
     // the last convolution with the 1x1 head fused into its epilogue (an LDS
     // tile over all 32 channels: the autotuned one, else the first offered)
     ConvArgs c{};
@@ -300,13 +339,7 @@ class Net {
     c.cout = 32;
     c.relu = 1;
     c.split = 32;
-    c.tile = fwd_tile[9];
-    if (!conv3x3_tile_lds(32, c.tile))
-      for (int t = 0; t < conv3x3_tiles(32); t++)
-        if (conv3x3_tile_lds(32, t)) {
-          c.tile = t;
-          break;
-        }
+    c.tile = lds_tile32(9);
     c.head_w = params + head_w_off;
     c.head_b = params + head_b_off;
     c.head_noisy = (const float4 *)noisy.ptr;

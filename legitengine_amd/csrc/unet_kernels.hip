@@ -61,6 +61,48 @@ LGX_D bf16x8 load8(const ConvInput &in, int n, int y, int x, int h, int w, int c
   return *(const bf16x8 *)(in.b + (((size_t)n * h + y) * w + x) * in.cb + (c - in.ca));
 }
 
+// Stores one output row of a wave's register tile: PB blocks of 16 pixels
+// (x0 + pb * 16 + col) x MB blocks of 16 channels (co0 + mb * 16 + kg * 4,
+// 4 per lane): bias (bias_r, this lane's channels), ReLU, the optional
+// mask (four channels in one read), the split into out0 / out1.
+template <int PB, int MB>
+LGX_D void conv_store_row(const ConvArgs &a, const f32x4 (&acc)[PB][MB], const float (&bias_r)[MB][4], int n, int y,
+                          int x0, int col, int kg, int co0) {
+  typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int pb = 0; pb < PB; pb++) {
+    const int x = x0 + pb * 16 + col;
+    if (x >= a.w) continue;
+    const size_t pix = ((size_t)n * a.h + y) * a.w + x;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+      const int co = co0 + mb * 16 + kg * 4;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        v[i] = acc[pb][mb][i];
+        if (a.bias) v[i] += bias_r[mb][i];
+        if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
+      }
+      // split is a multiple of 16 (checked on the host), so the half is the
+      // same for the whole 16-channel block: a uniform choice of pointers
+      // (a per-lane choice would become a per-lane read of the arguments)
+      const bool lo = co0 + mb * 16 < a.split;
+      bf16 *const dst = lo ? a.out0 : a.out1;
+      const bf16 *const mask = lo ? a.mask0 : a.mask1;
+      const int stride = lo ? a.split : a.cout - a.split, cch = lo ? co : co - a.split;
+      if (mask) {
+        const bf16x4 m = *(const bf16x4 *)(mask + pix * stride + cch);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (!(bf2f(m[i]) > 0.f)) v[i] = 0.f;
+      }
+      bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
+      *(bf16x4 *)(dst + pix * stride + cch) = o;
+    }
+  }
+}
+
 // Forward 3x3 convolution (stride 1, zero padding 1) + bias + ReLU, and the
 // same kernel with flipped/transposed weights for the input gradient.
 // A wave computes R output rows x PB blocks of 16 pixels x all output
@@ -76,6 +118,11 @@ LGX_D bf16x8 load8(const ConvInput &in, int n, int y, int x, int h, int w, int c
 template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_kernel(const ConvArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = blockIdx.z, y0 = blockIdx.y * R, x0 = (blockIdx.x * 4 + wave) * (PB * 16);
+  // the bias, in LDS for the epilogue (a global read placed after the
+  // epilogue's stores would wait for their round trip: they might alias it)
+  __shared__ __attribute__((aligned(16))) float s_bias[MB * 16];
+  if ((int)threadIdx.x < MB * 16) s_bias[threadIdx.x] = a.bias ? a.bias[threadIdx.x] : 0.f;
+  __syncthreads();
   if (x0 >= a.w) return;
   const int col = lane & 15, kg = lane >> 4;
   const int cin = a.ca + a.cb, chunks = cin >> 5;
@@ -88,8 +135,8 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
       for (int mb = 0; mb < MB; mb++) acc[r][pb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int cc = 0; cc < chunks; cc++) {
     // the chunk's source is uniform: A (optionally at half resolution) or B
-    const bool from_a = cc * 32 < a.ca;
-    const bool up = from_a && a.a_up;
+      const bool from_a = cc * 32 < a.ca;
+      const bool up = from_a && a.a_up;
     const int cs = from_a ? a.ca : a.cb;
     const int sw = up ? (a.w >> 1) : a.w, sh = up ? (a.h >> 1) : a.h;
     const bf16 *img = (from_a ? a.a : a.b) + (size_t)n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca) + kg * 8;
@@ -134,49 +181,23 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
       }
     }
   }
+  // Epilogue.  Masks are read into registers before the stores they feed:
+  // for all the compiler knows a store may alias them, so a read placed
+  // after a store would wait for that store's memory round trip.
+  float bias_r[MB][4];
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++) {
+    const float4 b4 = *(const float4 *)(&s_bias[mb * 16 + kg * 4]);
+    bias_r[mb][0] = b4.x;
+    bias_r[mb][1] = b4.y;
+    bias_r[mb][2] = b4.z;
+    bias_r[mb][3] = b4.w;
+  }
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int y = y0 + r;
     if (y >= a.h) break;
-#pragma unroll
-    for (int pb = 0; pb < PB; pb++) {
-      const int x = x0 + pb * 16 + col;
-      if (x >= a.w) continue;
-      const size_t pix = ((size_t)n * a.h + y) * a.w + x;
-#pragma unroll
-      for (int mb = 0; mb < MB; mb++) {
-        const int co = mb * 16 + kg * 4;
-        float v[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          v[i] = acc[r][pb][mb][i];
-          if (a.bias) v[i] += a.bias[co + i];
-          if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
-        }
-        bf16 *dst;
-        const bf16 *mask;
-        int stride, cch;
-        if (co < a.split) {
-          dst = a.out0;
-          mask = a.mask0;
-          stride = a.split;
-          cch = co;
-        } else {
-          dst = a.out1;
-          mask = a.mask1;
-          stride = a.cout - a.split;
-          cch = co - a.split;
-        }
-        if (mask) {
-#pragma unroll
-          for (int i = 0; i < 4; i++)
-            if (!(bf2f(mask[pix * stride + cch + i]) > 0.f)) v[i] = 0.f;
-        }
-        typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
-        bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
-        *(bf16x4 *)(dst + pix * stride + cch) = o;
-      }
-    }
+    conv_store_row<PB, MB>(a, acc[r], bias_r, n, y, x0, col, kg, 0);
   }
 }
 
@@ -195,20 +216,95 @@ template <int MB, int PB, int R> __global__ __launch_bounds__(256) void conv3x3_
 // needs MB = 2, i.e. all 32 channels of a pixel in one workgroup).
 // With one input chunk and no channel split every tile uses the same
 // weights: they stay in LDS from the workgroup's first tile on.
-template <int MB, int WR, int WP, bool HEAD>
-__global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x, int tiles_y, int splits) {
+// Raw G-buffer texels of one pixel for the G-buffer-fed first convolution
+// (GIN = 1: RG16F normal, RGBA8 albedo; GIN = 2: RG32F, RGBA32F), kept raw
+// in registers while in flight and converted when staged.
+template <int GIN> struct GinRaw;
+template <> struct GinRaw<1> {
+  float4 c;
+  float d;
+  uint32_t n, al;
+};
+template <> struct GinRaw<2> {
+  float4 c;
+  float d;
+  float2 n;
+  float4 al;
+};
+template <int GIN> LGX_D GinRaw<GIN> gin_load(const ConvArgs &a, int y, int x) {
+  GinRaw<GIN> r;
+  r.c = a.gin_noisy[(size_t)y * a.gin_noisy_pitch + x];
+  r.d = a.gin_depth[(size_t)y * a.gin_depth_pitch + x];
+  if constexpr (GIN == 1) {
+    r.n = ((const uint32_t *)a.gin_normal)[(size_t)y * a.gin_normal_pitch + x];
+    r.al = ((const uint32_t *)a.gin_albedo)[(size_t)y * a.gin_albedo_pitch + x];
+  } else {
+    r.n = ((const float2 *)a.gin_normal)[(size_t)y * a.gin_normal_pitch + x];
+    r.al = ((const float4 *)a.gin_albedo)[(size_t)y * a.gin_albedo_pitch + x];
+  }
+  return r;
+}
+// The first 16 channels of the pixel's record, as pack_gbuffer32_kernel packs them.
+template <int GIN> LGX_D void gin_pack(const GinRaw<GIN> &r, bf16x8 &v0, bf16x8 &v1) {
+  float nx, ny, ar, ag, ab;
+  if constexpr (GIN == 1) {
+    nx = h2f((uint16_t)(r.n & 0xffffu));
+    ny = h2f((uint16_t)(r.n >> 16));
+    const float k = 1.0f / 255.0f;
+    ar = (float)(r.al & 0xffu) * k;
+    ag = (float)((r.al >> 8) & 0xffu) * k;
+    ab = (float)((r.al >> 16) & 0xffu) * k;
+  } else {
+    nx = r.n.x;
+    ny = r.n.y;
+    ar = r.al.x;
+    ag = r.al.y;
+    ab = r.al.z;
+  }
+  v0 = bf16x8{f2bf(r.c.x), f2bf(r.c.y), f2bf(r.c.z), f2bf(r.d), f2bf(nx), f2bf(ny), f2bf(ar), f2bf(ag)};
+  v1 = bf16x8{};
+  v1[0] = f2bf(ab);
+}
+
+// LDS bytes of conv3x3_lds_kernel<MB, WR, WP> (the layout described in it),
+// and the workgroups per CU that LDS allows, at most 3 (168 VGPRs; 4 would
+// leave 128, fewer than the tiles' accumulators and operands need): the
+// register allocation is held to that occupancy.
+constexpr int conv_lds_bytes(int MB, int WR, int WP) {
+  return (4 * (((4 * WR + 2) * (16 * WP + 2) * 8 + 127) / 128 * 128) + 4 * 9 * MB * 16 * 8) * 2;
+}
+constexpr int conv_lds_blocks(int MB, int WR, int WP) {
+  return 160 * 1024 / conv_lds_bytes(MB, WR, WP) < 3 ? 160 * 1024 / conv_lds_bytes(MB, WR, WP) : 3;
+}
+
+template <int MB, int WR, int WP, bool HEAD, int GIN = 0>
+__global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x,
+                                                                                      int tiles_y, int splits) {
   constexpr int TH = 4 * WR, TW = 16 * WP, IW = TW + 2, IH = TH + 2;
-  constexpr int IN_PIECES = IH * IW * 4;     // 16-byte pieces of the input tile
-  constexpr int W_PIECES = 9 * MB * 16 * 4;  // of the chunk's weights
+  constexpr int NPIX = IH * IW;                       // input tile pixels (with the halo)
+  constexpr int IN_PIECES = (NPIX + 15) / 16 * 16 * 4;  // 16-byte pieces, pixels rounded up to 16
+  constexpr int W_PIECES = 9 * MB * 16 * 4;            // of the chunk's weights
   constexpr int IN_PER = (IN_PIECES + 255) / 256, W_PER = (W_PIECES + 255) / 256;
-  // LDS layout [channel group kg][pixel or weight row][8 channels]: the 8
-  // consecutive lanes an LDS access serves together (same kg, consecutive
-  // pixels / rows) read 128 contiguous bytes, free of bank conflicts; the
-  // weight planes are padded by 64 bytes so the stores spread over banks too.
-  constexpr int PLANE_IN = IH * IW * 8, PLANE_W = 9 * MB * 16 * 8 + 32;
+  // LDS layout [channel group kg][pixel or weight row][8 channels], every
+  // plane a multiple of 256 bytes (the 64 banks).  A 128-bit LDS access is
+  // served 16 lanes at a time, each 8-lane group of a wave with one from
+  // the other 16-lane half of its 32 (counters: 2-way conflicts when planes
+  // started 16 banks apart, none when 8-lane group g covers bank half
+  // g & 1).  Matrix-operand reads (lanes: kg = lane / 16, pixel = lane % 16)
+  // therefore put 8-lane groups of different planes on opposite bank halves,
+  // and the staging stores use the same rule (see piece()).
+  constexpr int PLANE_IN = (NPIX * 8 + 127) / 128 * 128, PLANE_W = 9 * MB * 16 * 8;
   __shared__ __attribute__((aligned(16))) bf16 s_in[4 * PLANE_IN];
   __shared__ __attribute__((aligned(16))) bf16 s_w[4 * PLANE_W];
+  static_assert(sizeof(s_in) + sizeof(s_w) == conv_lds_bytes(MB, WR, WP), "LDS layout");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // 16-byte piece q of a tile -> (pixel or weight row, part = channel group):
+  // a wave's 64 pieces are 16 consecutive pixels x 4 parts, lane l taking
+  // pixel l % 16 of part l / 16 -- the lane pattern of the operand reads
+  auto piece = [](int q, int &pix, int &part) {
+    pix = ((q >> 6) << 4) + (q & 15);
+    part = (q >> 4) & 3;
+  };
   const int col = lane & 15, kg = lane >> 4;
   const int chunks = (a.ca + a.cb) >> 5;
   const size_t wtap = (size_t)chunks * a.cout * 32;
@@ -234,31 +330,48 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
   bf16x8 pin[IN_PER], pw[W_PER];
   uint32_t valid = 0;  // bit i: piece i of pin lies in the image
   const bool w_resident = !HEAD && chunks == 1 && splits == 1;
+  // G-buffer-fed input: one thread per halo pixel, raw texels in flight
+  constexpr int GIN_PER = GIN ? (IH * IW + 255) / 256 : 1;
+  GinRaw<GIN ? GIN : 1> graw[GIN_PER];
   auto fetch = [&](const Tile &tl, int cc, bool with_w) {
-    const bool from_a = cc * 32 < a.ca;
-    const bool up = from_a && a.a_up;
-    const int cs = from_a ? a.ca : a.cb;
-    const int sw = up ? (a.w >> 1) : a.w, sh = up ? (a.h >> 1) : a.h;
-    const bf16 *img = (from_a ? a.a : a.b) + (size_t)tl.n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca);
-    valid = 0;
+    if constexpr (GIN) {
+      valid = 0;
 #pragma unroll
-    for (int i = 0; i < IN_PER; i++) {
-      const int q = min(tid + i * 256, IN_PIECES - 1);
-      const int p = q >> 2, part = q & 3;
-      const int y = tl.y0 - 1 + p / IW, x = tl.x0 - 1 + p % IW;
-      const bool ok = y >= 0 && y < a.h && x >= 0 && x < a.w;
-      const int yc = min(max(y, 0), a.h - 1), xc = min(max(x, 0), a.w - 1);
-      const uint32_t off = (uint32_t)((up ? (yc >> 1) : yc) * sw + (up ? (xc >> 1) : xc)) * (uint32_t)cs + part * 8;
-      pin[i] = *(const bf16x8 *)(img + off);
-      valid |= (uint32_t)ok << i;
+      for (int i = 0; i < GIN_PER; i++) {
+        const int p = min(tid + i * 256, IH * IW - 1);
+        const int y = tl.y0 - 1 + p / IW, x = tl.x0 - 1 + p % IW;
+        const bool ok = y >= 0 && y < a.h && x >= 0 && x < a.w;
+        graw[i] = gin_load<GIN>(a, min(max(y, 0), a.h - 1), min(max(x, 0), a.w - 1));
+        valid |= (uint32_t)ok << i;
+      }
+    } else {
+      const bool from_a = cc * 32 < a.ca;
+      const bool up = from_a && a.a_up;
+      const int cs = from_a ? a.ca : a.cb;
+      const int sw = up ? (a.w >> 1) : a.w, sh = up ? (a.h >> 1) : a.h;
+      const bf16 *img = (from_a ? a.a : a.b) + (size_t)tl.n * sh * sw * cs + (from_a ? cc * 32 : cc * 32 - a.ca);
+      valid = 0;
+#pragma unroll
+      for (int i = 0; i < IN_PER; i++) {
+        int p, part;
+        piece(min(tid + i * 256, IN_PIECES - 1), p, part);
+        p = min(p, NPIX - 1);  // a rounding piece: loaded in bounds, never staged
+        const int y = tl.y0 - 1 + p / IW, x = tl.x0 - 1 + p % IW;
+        const bool ok = y >= 0 && y < a.h && x >= 0 && x < a.w;
+        const int yc = min(max(y, 0), a.h - 1), xc = min(max(x, 0), a.w - 1);
+        const uint32_t off = (uint32_t)((up ? (yc >> 1) : yc) * sw + (up ? (xc >> 1) : xc)) * (uint32_t)cs + part * 8;
+        pin[i] = *(const bf16x8 *)(img + off);
+        valid |= (uint32_t)ok << i;
+      }
     }
     if (!with_w) return;
     const bf16 *wsrc = a.wt + ((size_t)cc * a.cout + tl.co0) * 32;
 #pragma unroll
     for (int i = 0; i < W_PER; i++) {
-      const int q = min(tid + i * 256, W_PIECES - 1);
-      const int tap = q / (MB * 16 * 4), rem = q % (MB * 16 * 4);
-      pw[i] = *(const bf16x8 *)(wsrc + tap * wtap + rem * 8);
+      int row, part;
+      piece(min(tid + i * 256, W_PIECES - 1), row, part);
+      const int tap = row / (MB * 16), co = row % (MB * 16);
+      pw[i] = *(const bf16x8 *)(wsrc + tap * wtap + (co * 4 + part) * 8);
     }
   };
   int t = blockIdx.x;
@@ -266,6 +379,10 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
   Tile cur = tile_of(t);
   fetch(cur, 0, true);
   bool stage_w = true;  // false once resident weights are staged
+  if constexpr (GIN) {  // record channels 16-31 are zero on every tile: staged once
+    const bf16x8 z = {};
+    for (int p = tid; p < 2 * IH * IW; p += 256) *(bf16x8 *)(s_in + (2 + p / (IH * IW)) * PLANE_IN + (p % (IH * IW)) * 8) = z;
+  }
   for (;;) {
     f32x4 acc[WR][WP][MB];
 #pragma unroll
@@ -274,18 +391,52 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
       for (int pb = 0; pb < WP; pb++)
 #pragma unroll
         for (int mb = 0; mb < MB; mb++) acc[r][pb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int cc = 0; cc < chunks; cc++) {
+    // The tile's bias, read before the matrix work and kept in registers: a
+    // global read placed after the tile's stores would wait for their memory
+    // round trip (for all the compiler knows they may alias it).  The read is
+    // unconditional like the tile loads (a layer without bias reads its
+    // weights instead and never adds the values).
+    float bias_r[MB][4];
+    if constexpr (!HEAD) {  // (the head's epilogue stores to LDS: it reads the bias there)
+      const float *bsrc = a.bias ? a.bias : (const float *)a.wt;
 #pragma unroll
-      for (int i = 0; i < IN_PER; i++) {
-        const int q = tid + i * 256;
-        const bf16x8 z = {};
-        if (q < IN_PIECES) *(bf16x8 *)(s_in + (q & 3) * PLANE_IN + (q >> 2) * 8) = ((valid >> i) & 1u) ? pin[i] : z;
+      for (int mb = 0; mb < MB; mb++) {
+        const float4 b4 = *(const float4 *)(bsrc + cur.co0 + mb * 16 + kg * 4);
+        bias_r[mb][0] = b4.x;
+        bias_r[mb][1] = b4.y;
+        bias_r[mb][2] = b4.z;
+        bias_r[mb][3] = b4.w;
+      }
+    }
+    for (int cc = 0; cc < chunks; cc++) {
+      if constexpr (GIN) {
+#pragma unroll
+        for (int i = 0; i < GIN_PER; i++) {
+          const int p = tid + i * 256;
+          bf16x8 v0 = {}, v1 = {};
+          if ((valid >> i) & 1u) gin_pack<GIN>(graw[i], v0, v1);
+          if (p < IH * IW) {
+            *(bf16x8 *)(s_in + p * 8) = v0;
+            *(bf16x8 *)(s_in + PLANE_IN + p * 8) = v1;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < IN_PER; i++) {
+          const int q = tid + i * 256;
+          const bf16x8 z = {};
+          int p, part;
+          piece(q, p, part);
+          if (q < IN_PIECES && p < NPIX) *(bf16x8 *)(s_in + part * PLANE_IN + p * 8) = ((valid >> i) & 1u) ? pin[i] : z;
+        }
       }
       if (stage_w) {
 #pragma unroll
         for (int i = 0; i < W_PER; i++) {
           const int q = tid + i * 256;
-          if (q < W_PIECES) *(bf16x8 *)(s_w + (q & 3) * PLANE_W + (q >> 2) * 8) = pw[i];
+          int row, part;
+          piece(q, row, part);
+          if (q < W_PIECES) *(bf16x8 *)(s_w + part * PLANE_W + row * 8) = pw[i];
         }
         stage_w = !w_resident;
       }
@@ -365,50 +516,55 @@ __global__ __launch_bounds__(256, HEAD ? 3 : 2) void conv3x3_lds_kernel(const Co
         }
       }
       __syncthreads();  // the next tile's stash overwrites the activations
-    } else
+    } else {
+    if (a.pool_out) {
+      // 2x2 average of the stored (BF16) outputs: rows r, r + 1 are this
+      // lane's, columns x, x + 1 are lanes col, col + 1 (same channel group);
+      // summed in avgpool_kernel's order (y, x), (y, x+1), (y+1, x), (y+1, x+1)
+      const int hh = a.h >> 1, ww = a.w >> 1;
+#pragma unroll
+      for (int r = 0; r + 1 < WR; r += 2) {
+        const int y = cur.y0 + wave * WR + r;  // even
+#pragma unroll
+        for (int pb = 0; pb < WP; pb++) {
+          const int x = cur.x0 + pb * 16 + col;
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) {
+            const int co = cur.co0 + mb * 16 + kg * 4;
+            float t[4], b[4], tn[4], bn[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              float v0 = acc[r][pb][mb][i], v1 = acc[r + 1][pb][mb][i];
+              if (a.bias) {
+                v0 += bias_r[mb][i];
+                v1 += bias_r[mb][i];
+              }
+              if (a.relu) {
+                v0 = v0 > 0.f ? v0 : 0.f;
+                v1 = v1 > 0.f ? v1 : 0.f;
+              }
+              t[i] = bf2f(f2bf(v0));
+              b[i] = bf2f(f2bf(v1));
+              tn[i] = __shfl_xor(t[i], 1);
+              bn[i] = __shfl_xor(b[i], 1);
+            }
+            if (!(col & 1) && (y >> 1) < hh && (x >> 1) < ww) {
+              typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+              bf16x4 o;
+#pragma unroll
+              for (int i = 0; i < 4; i++) o[i] = f2bf((((t[i] + tn[i]) + b[i]) + bn[i]) * 0.25f);
+              *(bf16x4 *)(a.pool_out + (((size_t)cur.n * hh + (y >> 1)) * ww + (x >> 1)) * a.cout + co) = o;
+            }
+          }
+        }
+      }
+    }
 #pragma unroll
     for (int r = 0; r < WR; r++) {
       const int y = cur.y0 + wave * WR + r;
       if (y >= a.h) break;
-#pragma unroll
-      for (int pb = 0; pb < WP; pb++) {
-        const int x = cur.x0 + pb * 16 + col;
-        if (x >= a.w) continue;
-        const size_t pix = ((size_t)cur.n * a.h + y) * a.w + x;
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++) {
-          const int co = cur.co0 + mb * 16 + kg * 4;
-          float v[4];
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            v[i] = acc[r][pb][mb][i];
-            if (a.bias) v[i] += a.bias[co + i];
-            if (a.relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
-          }
-          bf16 *dst;
-          const bf16 *mask;
-          int stride, cch;
-          if (co < a.split) {
-            dst = a.out0;
-            mask = a.mask0;
-            stride = a.split;
-            cch = co;
-          } else {
-            dst = a.out1;
-            mask = a.mask1;
-            stride = a.cout - a.split;
-            cch = co - a.split;
-          }
-          if (mask) {
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-              if (!(bf2f(mask[pix * stride + cch + i]) > 0.f)) v[i] = 0.f;
-          }
-          typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
-          bf16x4 o = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
-          *(bf16x4 *)(dst + pix * stride + cch) = o;
-        }
-      }
+      conv_store_row<WP, MB>(a, acc[r], bias_r, cur.n, y, cur.x0, col, kg, cur.co0);
+    }
     }
     t += gridDim.x;
     if (t >= T) break;
@@ -956,6 +1112,13 @@ int conv3x3_tile_lds(int cout, int tile) {
   return t && tile >= 0 && tile < n && t[tile].lds && t[tile].mbw * 16 == cout;
 }
 
+int conv3x3_tile_pool(int cout, int tile) {
+  int n = 0;
+  if (cout % 16) return 0;
+  const ConvTile *t = conv_tiles(cout / 16, &n);
+  return t && tile >= 0 && tile < n && t[tile].lds && t[tile].r % 2 == 0;
+}
+
 int conv3x3_tiles(int cout) {
   int n = 0;
   if (cout % 16) return 0;
@@ -988,6 +1151,16 @@ template <int MBW, int WR, int WP> static void launch_conv_lds(const ConvArgs &a
   if constexpr (MBW == 2) {
     if (a.head_out) {  // one tile per workgroup: nothing is prefetched across the head's epilogue
       hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, true>), dim3((unsigned)tiles), dim3(256), 0, s, a, tx, ty,
+                         splits);
+      return;
+    }
+    if (a.gin_fmt == 1) {
+      hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, false, 1>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty,
+                         splits);
+      return;
+    }
+    if (a.gin_fmt == 2) {
+      hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, false, 2>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty,
                          splits);
       return;
     }
@@ -1030,6 +1203,12 @@ template <int MB> static hipError_t conv_mb(const ConvArgs &a, const ConvTile &t
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
   if (a.cout % 16) return hipErrorInvalidValue;
   if (a.head_out && (a.cout != 32 || a.n != 1 || !conv3x3_tile_lds(a.cout, a.tile))) return hipErrorInvalidValue;
+  if (a.gin_fmt && (a.gin_fmt > 2 || a.head_out || a.cout != 32 || a.n != 1 || a.ca != 32 || a.cb ||
+                    !conv3x3_tile_lds(a.cout, a.tile)))
+    return hipErrorInvalidValue;
+  if (a.pool_out && (a.head_out || a.mask0 || a.mask1 || !conv3x3_tile_pool(a.cout, a.tile)))
+    return hipErrorInvalidValue;
+  if (a.split < 0 || a.split > a.cout || a.split % 16) return hipErrorInvalidValue;  // whole 16-channel blocks
   const int mb = a.cout / 16;
   int n = 0;
   const ConvTile *tiles = conv_tiles(mb, &n);

@@ -197,6 +197,7 @@ def main(argv=None):
     t.add_argument("--schedule", default="cosine", choices=("cosine", "const"))
     t.add_argument("--loss", default="l1", choices=("l1", "l2"))
     t.add_argument("--grid", type=int, nargs="+", default=None, help="pattern width [height]")
+    t.add_argument("--dataset-seed", type=int, default=1000, help="first scene seed of the training pairs")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", default="200,201,202,203,204,205,206,207")
@@ -210,7 +211,8 @@ def main(argv=None):
 
     if args.cmd == "train":
         args.grid = NOISY_GRID if args.grid is None else (args.grid[0] if len(args.grid) == 1 else tuple(args.grid))
-        X, T = build_dataset(args.scenes, args.poses, log=log, footprints=(args.footprint,), grid=args.grid)
+        X, T = build_dataset(args.scenes, args.poses, seed0=args.dataset_seed, log=log, footprints=(args.footprint,),
+                             grid=args.grid)
         X = X[args.footprint]
         net = dev.UNet(args.batch, CROP, CROP, seed=0)
         if args.resume:
@@ -232,7 +234,8 @@ def main(argv=None):
                 json.dump({"history": hist, "train_seconds": secs, "steps": args.steps, "batch": args.batch,
                            "scenes": args.scenes, "poses": args.poses, "footprint": args.footprint,
                            "schedule": args.schedule, "loss": args.loss, "lr": args.lr, "grid": args.grid,
-                           "resume": args.resume, "eval": rows}, f)
+                           "resume": args.resume, "dataset_seed": args.dataset_seed,
+                           "eval": rows}, f)
     else:
         args.grid = NOISY_GRID if args.grid is None else (args.grid[0] if len(args.grid) == 1 else tuple(args.grid))
         net = dev.UNet(1, H, W)

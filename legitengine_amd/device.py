@@ -431,7 +431,7 @@ class Batch:
     views: (GBufferDev, view, proj) tuples; net: UNet or None."""
 
     def __init__(self, views, dirs=4, max_steps=64, levels=6, sampler=lgx.SAMPLER_HIZ_POINT, grid=4,
-                 footprint_dirs=0, tile=0, workgroups=0, net=None):
+                 footprint_dirs=0, tile=0, workgroups=0, net=None, chunks=0):
         n = len(views)
         vs = (lgx.BatchView * n)()
         for i, (gb, view, proj) in enumerate(views):
@@ -442,7 +442,7 @@ class Batch:
         c = lgx.BatchConfig()
         c.dirs, c.max_steps, c.levels, c.sampler = dirs, max_steps, levels, sampler
         c.grid, c.grid_y = lgx.pattern(grid)
-        c.footprint_dirs, c.tile, c.workgroups = footprint_dirs, tile, workgroups
+        c.footprint_dirs, c.tile, c.workgroups, c.chunks = footprint_dirs, tile, workgroups, chunks
         self._keep = (views, net)
         self.n = n
         h = C.c_void_p()

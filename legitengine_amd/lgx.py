@@ -84,7 +84,7 @@ class BatchView(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [("dirs", C.c_int32), ("max_steps", C.c_int32), ("levels", C.c_int32), ("grid", C.c_int32),
                 ("sampler", C.c_int32), ("footprint_dirs", C.c_int32), ("tile", C.c_int32),
-                ("workgroups", C.c_int32), ("grid_y", C.c_int32)]
+                ("workgroups", C.c_int32), ("grid_y", C.c_int32), ("chunks", C.c_int32)]
 
 # Every entry point declared in include/lgx.h, with its ctypes signature.
 SIGNATURES = {

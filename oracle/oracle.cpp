@@ -33,10 +33,20 @@
  *                         clamp-to-edge, level selection
  *   gather                planar KAT (uniform radiance plane gives
  *                         ambient * sum_d (2/D) h(0, theta_max_d), exactly ambient
- *                         where n = eye), zero-radiance = ambient-only, linearity
- *                         of the (L - ambient) part in radiance, float vs double
+ *                         where n = eye: the no-hit branch), zero-radiance =
+ *                         ambient-only, linearity of the (L - ambient) part in
+ *                         radiance, float vs double; the hit branch (:254-264)
+ *                         by a sphere of constant radial depth around the camera,
+ *                         where every on-screen sample rises above the horizon at
+ *                         theta = atan2(cos((phi - delta)/2), sin(phi/2)) in
+ *                         closed form: the hit sum telescopes (ring of ambient
+ *                         radiance) or is summed hit by hit with the side weights,
+ *                         the slice integral by quadrature (tests/kat_scenes.py)
  *   bilateral             constant image fixed point, r=0 identity, weights
- *                         normalised (output inside the tap range)
+ *                         normalised (output inside the tap range); a two-region
+ *                         depth/normal field whose cross-region taps weigh
+ *                         exp(-10.5 |dd|) exp(-20 |dn|^2) = 0.266, every output
+ *                         column a closed-form ratio of tap counts
  *   regression denoiser   affine colour-in-depth reproduced, radius 0 passthrough
  * Parity with the reference's own GPU output is unpinned: the reference runs
  * only under Vulkan, which this environment lacks (no loader, no GPU, empty

@@ -95,6 +95,7 @@ def slice_h(tau_n, a, b, order=64):
 def sphere_scene(w=80, h=60, r=2.0, radiance=0.5, ring_ambient=True):
     """G-buffer and one-level pyramid of the sphere scene.
 
+    radiance: one value for rgb, or three (the closed form is per channel).
     ring_ambient: the radiance is `radiance` on the texels whose samples keep
     the full side weight (x in [0.1 w, 0.9 w), y in [0.1 h, 0.9 h)) and the
     ambient value on the outer ring, where the side fade (:220-228) is below
@@ -110,8 +111,10 @@ def sphere_scene(w=80, h=60, r=2.0, radiance=0.5, ring_ambient=True):
     normal = np.moveaxis(-rc, 0, -1)
     octn = oct_encode(normal)
     depth = np.full((h, w), r, np.float32)
-    rad = np.full((h, w, 4), AMBIENT if ring_ambient else radiance, np.float32)
-    if ring_ambient:
+    rad = np.full((h, w, 4), AMBIENT, np.float32)
+    if not ring_ambient:
+        rad[..., :3] = radiance
+    else:
         rad[h // 10:h - h // 10, w // 10:w - w // 10, :3] = radiance
     rad[..., 3] = 1.0
     return dict(view=view, proj=proj, depth=depth, oct=octn, rad=rad, w=w, h=h, r=r, radiance=radiance,
@@ -119,9 +122,12 @@ def sphere_scene(w=80, h=60, r=2.0, radiance=0.5, ring_ambient=True):
 
 
 def sphere_expected(sc, px, py, dirs, footprint_dirs):
-    """Gather output (rgb, equal channels) of pixel (px, py) on the 1x1
-    pattern (Hammersley(0, 1) = (0, 0): directions 2 pi d / D, no linear
-    offset), from the closed forms above."""
+    """Gather output of pixel (px, py) on the 1x1 pattern (Hammersley(0, 1)
+    = (0, 0): directions 2 pi d / D, no linear offset), from the closed
+    forms above: a float for one radiance value, rgb for three."""
+    if np.ndim(sc["radiance"]):
+        return np.array([sphere_expected(dict(sc, radiance=c), px, py, dirs, footprint_dirs)
+                         for c in sc["radiance"]])
     w, h, proj = sc["w"], sc["h"], sc["proj"]
     L, amb = float(np.float32(sc["radiance"])), AMBIENT
     cx, cy = px + 0.5, py + 0.5
@@ -252,27 +258,32 @@ def known_scene(lgx, with_objects=True):
     for k in range(3):
         s.light_dir[k] = ld[k]
     s.light_intensity = 2.0
-    for k, c in enumerate((0.55, 0.85, 0.08)):
+    for k, c in enumerate((0.5, 0.8, 0.05)):
         s.cam_pos[k] = c
-    s.cam_hor, s.cam_vert = math.pi, -0.75
+    s.cam_hor, s.cam_vert = 0.0, 0.6  # looking along +z, tilted down
     s.fovy, s.znear, s.zfar = math.radians(60.0), 0.01, 1000.0
     return s
 
 
-def _box_hit(o, d, mn, mx):
-    """Slab test, vectorised: (tn, tf) per ray and the entry axis."""
-    inv = 1.0 / np.where(d == 0, 1e-300, d)
-    t0 = (np.asarray(mn)[:, None] - o[:, None] if o.ndim == 1 else np.asarray(mn)[:, None] - o) * inv
-    t1 = (np.asarray(mx)[:, None] - o[:, None] if o.ndim == 1 else np.asarray(mx)[:, None] - o) * inv
+def _slabs(o, d, mn, mx):
+    """Slab test per ray (columns of d; o one point or one per ray): the
+    entry and exit parameters, the entry axis, and the gap between the two
+    largest entry parameters (small near a box edge)."""
+    o = o if o.ndim == 2 else o[:, None]
+    dd = np.where(d == 0, 1e-300, d)
+    t0 = (np.asarray(mn, np.float64)[:, None] - o) / dd
+    t1 = (np.asarray(mx, np.float64)[:, None] - o) / dd
     lo, hi = np.minimum(t0, t1), np.maximum(t0, t1)
-    return lo.max(0), hi.min(0), lo.argmax(0)
+    slo = np.sort(lo, 0)
+    return lo.max(0), hi.min(0), lo.argmax(0), slo[2] - slo[1]
 
 
-def analytic_gbuffer(lgx, s, w, h):
+def analytic_gbuffer(lgx, s, w, h, margin=1e-3):
     """Float64 ray caster of the scene: per pixel the radial depth, world
-    normal, albedo, direct radiance E max(0, n.l) vis, and a mask of pixels
-    that sit clear of every silhouette and shadow boundary (margin 1e-3 in
-    the ray parameters), where float32 rendering cannot pick differently."""
+    normal, albedo and direct radiance E max(0, n.l) vis (shadow rays test
+    the boxes and spheres, not the room), plus a mask of the pixels that sit
+    clear of every silhouette, edge and shadow boundary by `margin` in the
+    ray parameters, where float32 rendering cannot decide differently."""
     view, proj = lgx.scene_matrices(s, w, h)
     V = np.asarray(view, np.float64).reshape(4, 4).T
     P = np.asarray(proj, np.float64).reshape(4, 4).T
@@ -284,69 +295,68 @@ def analytic_gbuffer(lgx, s, w, h):
     d = v[:3] / v[3] - cam[:, None]
     d /= np.linalg.norm(d, axis=0)
     n_px = d.shape[1]
-    # room (inside of the unit cube)
-    tw = np.where(d > 0, (1 - cam[:, None]) / np.where(d == 0, 1, d), (0 - cam[:, None]) / np.where(d == 0, 1, d))
+    idx = np.arange(n_px)
+    # room: the inside of the unit cube
+    safe = np.where(d == 0, 1.0, d)
+    tw = np.where(d > 0, (1 - cam[:, None]) / safe, (0 - cam[:, None]) / safe)
     tw = np.where(d == 0, np.inf, tw)
     axis = tw.argmin(0)
     best = tw.min(0)
     srt = np.sort(tw, 0)
-    clear = (srt[1] - srt[0]) > 1e-3  # away from room edges
+    clear = (srt[1] - srt[0]) > margin
     normal = np.zeros((3, n_px))
-    normal[axis, np.arange(n_px)] = np.where(d[axis, np.arange(n_px)] > 0, -1.0, 1.0)
-    wall = axis * 2 + (d[axis, np.arange(n_px)] > 0)
-    alb = np.array([[s.wall_albedo[i][k] for k in range(3)] for i in range(6)])[wall].T.copy()
-    if s.n_boxes:
-        mn = [s.box_min[0][k] for k in range(3)]
-        mx = [s.box_max[0][k] for k in range(3)]
-        tn, tf, ax = _box_hit(cam, d, mn, mx)
+    normal[axis, idx] = np.where(d[axis, idx] > 0, -1.0, 1.0)
+    wall = axis * 2 + (d[axis, idx] > 0)
+    alb = np.array([[s.wall_albedo[i][k] for k in range(3)] for i in range(6)], np.float64)[wall].T.copy()
+    boxes = [([s.box_min[b][k] for k in range(3)], [s.box_max[b][k] for k in range(3)],
+              [s.box_albedo[b][k] for k in range(3)]) for b in range(s.n_boxes)]
+    spheres = [(np.array([s.sph_center[i][k] for k in range(3)], np.float64), float(s.sph_radius[i]),
+                [s.sph_albedo[i][k] for k in range(3)]) for i in range(s.n_spheres)]
+    for mn, mx, ab in boxes:
+        tn, tf, ax, gap = _slabs(cam, d, mn, mx)
         hit = (tn <= tf) & (tn > 1e-6) & (tn < best)
-        clear &= np.abs(tf - tn) > 1e-3
+        clear &= np.abs(tf - tn) > margin                  # silhouette
+        clear &= ~hit | (gap > margin)                    # box edge between two faces
+        clear &= ~((tn <= tf) & (np.abs(tn - best) < margin))  # depth tie with what is behind
         best = np.where(hit, tn, best)
         nb = np.zeros((3, n_px))
-        nb[ax, np.arange(n_px)] = np.where(d[ax, np.arange(n_px)] > 0, -1.0, 1.0)
-        lo = np.sort(np.minimum((np.asarray(mn)[:, None] - cam[:, None]) / d, (np.asarray(mx)[:, None] - cam[:, None]) / d), 0)
-        clear &= ~hit | ((lo[2] - lo[1]) > 1e-3)  # away from box edges
+        nb[ax, idx] = np.where(d[ax, idx] > 0, -1.0, 1.0)
         normal = np.where(hit, nb, normal)
-        alb = np.where(hit, np.array([[s.box_albedo[0][k]] for k in range(3)]), alb)
-    if s.n_spheres:
-        c = np.array([s.sph_center[0][k] for k in range(3)])
-        r = float(s.sph_radius[0])
+        alb = np.where(hit, np.array(ab, np.float64)[:, None], alb)
+    for c, r, ab in spheres:
         lv = cam - c
         b = lv @ d
         disc = b * b - (lv @ lv - r * r)
-        clear &= np.abs(disc) > 1e-3 * r
+        clear &= np.abs(disc) > margin * r
         t0 = -b - np.sqrt(np.maximum(disc, 0))
         hit = (disc >= 0) & (t0 > 1e-6) & (t0 < best)
+        clear &= ~((disc >= 0) & (np.abs(t0 - best) < margin))
         best = np.where(hit, t0, best)
         hp = cam[:, None] + d * best
-        ns = (hp - c[:, None]) / r
-        normal = np.where(hit, ns, normal)
-        alb = np.where(hit, np.array([[s.sph_albedo[0][k]] for k in range(3)]), alb)
-    # direct light with hard shadows from the box and the sphere (the room shell casts none)
+        normal = np.where(hit, (hp - c[:, None]) / r, normal)
+        alb = np.where(hit, np.array(ab, np.float64)[:, None], alb)
     l = np.array([s.light_dir[k] for k in range(3)], np.float64)
-    hp = cam[:, None] + d * best
+    ld = np.repeat(l[:, None], n_px, 1)
+    o = cam[:, None] + d * best + normal * 1e-4
     ndl = (normal * l[:, None]).sum(0)
     vis = np.ones(n_px)
-    o = hp + normal * 1e-4
-    if s.n_boxes:
-        tn, tf, _ = _box_hit(o, np.repeat(l[:, None], n_px, 1), mn, mx)
-        sh = (tn <= tf) & (tn > 1e-6)
-        clear &= ~(ndl > 0) | (np.abs(tf - tn) > 1e-3)
-        vis = np.where(sh, 0.0, vis)
-    if s.n_spheres:
+    lit_side = ndl > 0
+    for mn, mx, _ in boxes:
+        tn, tf, _, _ = _slabs(o, ld, mn, mx)
+        vis = np.where((tn <= tf) & (tn > 1e-6), 0.0, vis)
+        clear &= ~lit_side | (np.abs(tf - tn) > margin)
+    for c, r, _ in spheres:
         lv = o - c[:, None]
-        b = (lv * l[:, None]).sum(0)
+        b = (lv * ld).sum(0)
         disc = b * b - ((lv * lv).sum(0) - r * r)
         t1 = -b + np.sqrt(np.maximum(disc, 0))
-        sh = (disc >= 0) & (t1 > 1e-6)
-        clear &= ~(ndl > 0) | (np.abs(disc) > 1e-3 * r)
-        vis = np.where(sh, 0.0, vis)
-    clear &= np.abs(ndl) > 1e-3
+        vis = np.where((disc >= 0) & (t1 > 1e-6), 0.0, vis)
+        clear &= ~lit_side | (np.abs(disc) > margin * r)
+    clear &= np.abs(ndl) > margin
     e = float(s.light_intensity) * np.maximum(ndl, 0) * vis
-    rad = alb * e
-    sh_ = lambda a: np.moveaxis(a.reshape(-1, h, w), 0, -1)
-    return dict(depth=best.reshape(h, w), normal=sh_(normal), albedo=sh_(alb), radiance=sh_(rad),
-                clear=clear.reshape(h, w), lit=(e > 0).reshape(h, w))
+    img = lambda a: np.moveaxis(a.reshape(-1, h, w), 0, -1)
+    return dict(depth=best.reshape(h, w), normal=img(normal), albedo=img(alb), radiance=img(alb * e),
+                clear=clear.reshape(h, w), lit=(e > 0).reshape(h, w), shadowed=(lit_side & (vis == 0)).reshape(h, w))
 
 
 def decode_oct(o):

@@ -193,9 +193,11 @@ def test_register_tiles_bit_identical_and_autotune(dev):
 
 
 def test_fused_head_every_tile(dev):
-    """The inference head runs in the epilogue of the last convolution's LDS
-    tile (the autotuned one, or the first LDS tile when the chosen tile has
-    no LDS form). Every tile of that layer gives the unfused forward's
+    """Inference fuses both ends of the network into LDS convolution tiles:
+    the first convolution packs its input records from the G-buffer while
+    staging them, and the head runs in the epilogue of the last convolution
+    (each the autotuned tile, or the first LDS tile when the chosen tile has
+    no LDS form). Every tile of either layer gives the unfused forward's
     output."""
     from legitengine_amd import lgx
     sc = lgx.make_scene(6, 8, 2)
@@ -213,12 +215,13 @@ def test_fused_head_every_tile(dev):
     f0, d0 = net.tiles()
     n9 = lgx.lib().lgx_unet_conv_tiles(32)
     assert n9 >= 2
-    for t in range(n9):
-        f = list(f0)
-        f[9] = t
-        net.set_tiles(f, d0)
-        out = net.denoise(g, dn).numpy()
-        assert np.allclose(out, ref, rtol=0, atol=1e-6), (t, np.abs(out - ref).max())
+    for layer in (0, 9):
+        for t in range(n9):
+            f = list(f0)
+            f[layer] = t
+            net.set_tiles(f, d0)
+            out = net.denoise(g, dn).numpy()
+            assert np.allclose(out, ref, rtol=0, atol=1e-6), (layer, t, np.abs(out - ref).max())
 
 
 def test_denoise_rejects_bad_images_on_the_host(dev):
@@ -254,3 +257,4 @@ def test_denoise_rejects_bad_images_on_the_host(dev):
     assert L.lgx_unet_denoise(net.h, C.byref(g), C.byref(noisy), C.byref(bad), None) == lgx.EINVAL
     small = lgx.image(fake, w, h - 4, lgx.RGBA32F)
     assert L.lgx_unet_denoise(net.h, C.byref(g), C.byref(noisy), C.byref(small), None) == lgx.ESIZE
+

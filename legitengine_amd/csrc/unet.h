@@ -34,8 +34,9 @@ struct ConvArgs {
   // First convolution at inference (cout = 32, an LDS tile, n = 1): the
   // input records are packed from the noisy image and the G-buffer while
   // the tile is staged, channels (noisy rgb, depth, normal xy, albedo rgb,
-  // zeros), as pack_gbuffer32 would write them; a and b are unused.
-  int gin_fmt;                   // 0: off; 1: RG16F normal + RGBA8 albedo; 2: RG32F + RGBA32F
+  // zeros), the values pack_input gives for those records; a and b are
+  // unused.
+  int gin_fmt;  // 0: off; 1: RG16F normal + RGBA8 albedo; 2: RG32F + RGBA32F
   const float4 *gin_noisy;
   const float *gin_depth;
   const void *gin_normal, *gin_albedo;
@@ -69,9 +70,6 @@ int conv3x3_tile_pool(int cout, int tile);  // 1 if that tile can write the 2x2 
 hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s);
 size_t conv3x3_wgrad_scratch_bytes();
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s);
-hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12, hipStream_t s);
-// inference from the G-buffer: the BF16 input records and the head with the noisy image as residual
-hipError_t pack_gbuffer32(const lgx_image &noisy, const lgx_gbuffer &g, bf16 *x32, hipStream_t s);
 hipError_t head_fwd_img(const bf16 *d1, const lgx_image &noisy, const float *wh, const float *bh, int out_pitch,
                         float4 *out, hipStream_t s);
 hipError_t avgpool(const bf16 *x, int n, int h, int w, int c, bf16 *y, hipStream_t s);

@@ -244,7 +244,8 @@ template <int GIN> LGX_D GinRaw<GIN> gin_load(const ConvArgs &a, int y, int x) {
   }
   return r;
 }
-// The first 16 channels of the pixel's record, as pack_gbuffer32_kernel packs them.
+// The first 16 channels of the pixel's input record (pack_input_kernel's
+// values for the record the G-buffer and the noisy image give).
 template <int GIN> LGX_D void gin_pack(const GinRaw<GIN> &r, bf16x8 &v0, bf16x8 &v1) {
   float nx, ny, ar, ag, ab;
   if constexpr (GIN == 1) {
@@ -264,6 +265,13 @@ template <int GIN> LGX_D void gin_pack(const GinRaw<GIN> &r, bf16x8 &v0, bf16x8 
   v0 = bf16x8{f2bf(r.c.x), f2bf(r.c.y), f2bf(r.c.z), f2bf(r.d), f2bf(nx), f2bf(ny), f2bf(ar), f2bf(ag)};
   v1 = bf16x8{};
   v1[0] = f2bf(ab);
+}
+
+// The value of the neighbouring lane (lane ^ 1): a DPP quad permutation
+// [1, 0, 3, 2], a modifier of one VALU move (no LDS round trip as a shuffle
+// through ds_bpermute would take).
+LGX_D float lane_xor1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
 }
 
 // LDS bytes of conv3x3_lds_kernel<MB, WR, WP> (the layout described in it),
@@ -545,8 +553,8 @@ __global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_
               }
               t[i] = bf2f(f2bf(v0));
               b[i] = bf2f(f2bf(v1));
-              tn[i] = __shfl_xor(t[i], 1);
-              bn[i] = __shfl_xor(b[i], 1);
+              tn[i] = lane_xor1(t[i]);
+              bn[i] = lane_xor1(b[i]);
             }
             if (!(col & 1) && (y >> 1) < hh && (x >> 1) < ww) {
               typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -772,52 +780,6 @@ __global__ __launch_bounds__(256) void pack_input_kernel(const float *x12, size_
     v[0] = f2bf(x12[p * 12 + 8]);
   }
   *(bf16x8 *)(x32 + p * 32 + q * 8) = v;
-}
-
-// The same record straight from a G-buffer and the noisy irradiance image.
-template <int NFMT, int AFMT>
-__global__ __launch_bounds__(256) void pack_gbuffer_kernel(const float4 *noisy, int np_, const float *depth, int dp,
-                                                           const void *normal, int nrmp, const void *albedo, int ap,
-                                                           int w, int h, float *x12) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= w || y >= h) return;
-  const float4 c = noisy[(size_t)y * np_ + x];
-  const float4 nn = Texel<NFMT>::load(normal, (size_t)y * nrmp + x);
-  const float4 al = Texel<AFMT>::load(albedo, (size_t)y * ap + x);
-  float *o = x12 + ((size_t)y * w + x) * 12;
-  o[0] = c.x;
-  o[1] = c.y;
-  o[2] = c.z;
-  o[3] = depth[(size_t)y * dp + x];
-  o[4] = nn.x;
-  o[5] = nn.y;
-  o[6] = al.x;
-  o[7] = al.y;
-  o[8] = al.z;
-  o[9] = o[10] = o[11] = 0.f;
-}
-
-// Inference: the BF16 32-channel input straight from the G-buffer and the
-// noisy image (the values pack_gbuffer + pack_input produce, in one pass).
-template <int NFMT, int AFMT>
-__global__ __launch_bounds__(256) void pack_gbuffer32_kernel(const float4 *noisy, int np_, const float *depth, int dp,
-                                                             const void *normal, int nrmp, const void *albedo, int ap,
-                                                             int w, int h, bf16 *x32) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= w || y >= h) return;
-  const float4 c = noisy[(size_t)y * np_ + x];
-  const float4 nn = Texel<NFMT>::load(normal, (size_t)y * nrmp + x);
-  const float4 al = Texel<AFMT>::load(albedo, (size_t)y * ap + x);
-  bf16x8 v0 = {f2bf(c.x), f2bf(c.y), f2bf(c.z), f2bf(depth[(size_t)y * dp + x]), f2bf(nn.x), f2bf(nn.y),
-               f2bf(al.x), f2bf(al.y)};
-  bf16x8 v1 = {};
-  v1[0] = f2bf(al.z);
-  bf16x8 *o = (bf16x8 *)(x32 + ((size_t)y * w + x) * 32);
-  const bf16x8 z = {};
-  o[0] = v0;
-  o[1] = v1;
-  o[2] = z;
-  o[3] = z;
 }
 
 __global__ __launch_bounds__(256) void avgpool_kernel(const bf16 *x, int n, int h, int w, int c, bf16 *y) {
@@ -1281,36 +1243,6 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
 
 hipError_t pack_input(const float *x12, size_t npix, bf16 *x32, hipStream_t s) {
   hipLaunchKernelGGL(pack_input_kernel, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, s, x12, npix, x32);
-  return hipGetLastError();
-}
-hipError_t pack_gbuffer(const lgx_image &noisy, const lgx_gbuffer &g, float *x12, hipStream_t s) {
-  const int w = noisy.w, h = noisy.h;
-  dim3 grid((w + 63) / 64, (h + 3) / 4), block(256);
-  const float4 *np_ = (const float4 *)noisy.ptr;
-  const float *dp = (const float *)g.depth.ptr;
-  if (g.normal.format == LGX_RG16F && g.albedo.format == LGX_RGBA8)
-    hipLaunchKernelGGL((pack_gbuffer_kernel<LGX_RG16F, LGX_RGBA8>), grid, block, 0, s, np_, noisy.pitch, dp,
-                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x12);
-  else if (g.normal.format == LGX_RG32F && g.albedo.format == LGX_RGBA32F)
-    hipLaunchKernelGGL((pack_gbuffer_kernel<LGX_RG32F, LGX_RGBA32F>), grid, block, 0, s, np_, noisy.pitch, dp,
-                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x12);
-  else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-hipError_t pack_gbuffer32(const lgx_image &noisy, const lgx_gbuffer &g, bf16 *x32, hipStream_t s) {
-  const int w = noisy.w, h = noisy.h;
-  dim3 grid((w + 63) / 64, (h + 3) / 4), block(256);
-  const float4 *np_ = (const float4 *)noisy.ptr;
-  const float *dp = (const float *)g.depth.ptr;
-  if (g.normal.format == LGX_RG16F && g.albedo.format == LGX_RGBA8)
-    hipLaunchKernelGGL((pack_gbuffer32_kernel<LGX_RG16F, LGX_RGBA8>), grid, block, 0, s, np_, noisy.pitch, dp,
-                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x32);
-  else if (g.normal.format == LGX_RG32F && g.albedo.format == LGX_RGBA32F)
-    hipLaunchKernelGGL((pack_gbuffer32_kernel<LGX_RG32F, LGX_RGBA32F>), grid, block, 0, s, np_, noisy.pitch, dp,
-                       g.depth.pitch, g.normal.ptr, g.normal.pitch, g.albedo.ptr, g.albedo.pitch, w, h, x32);
-  else
-    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 hipError_t head_fwd_img(const bf16 *d1, const lgx_image &noisy, const float *wh, const float *bh, int out_pitch,

@@ -127,15 +127,49 @@ def lr_at(it, steps, lr, schedule):
     return lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * it / max(1, steps - 1))))
 
 
-def train(net, X, T, steps, batch=16, lr=1e-3, log=print, seed=0, log_every=50, schedule="cosine"):
+def crop_errors(net, X, T, batch=16):
+    """Relative squared error ||out - t||^2 / ||t||^2 of every training crop
+    under the current network (forward only)."""
+    err = torch.empty(X.shape[0], dtype=torch.float64, device="cuda")
+    for i in range(0, X.shape[0] - batch + 1, batch):
+        out = net.forward(X[i:i + batch])
+        d = (out[..., :3] - T[i:i + batch, ..., :3]).double()
+        t = T[i:i + batch, ..., :3].double()
+        err[i:i + batch] = (d * d).sum((1, 2, 3)) / ((t * t).sum((1, 2, 3)) + 1e-12)
+    rest = X.shape[0] % batch
+    if rest:  # the last, partial batch: padded with crops of the batch before it
+        i = X.shape[0] - batch
+        out = net.forward(X[i:])
+        d = (out[..., :3] - T[i:, ..., :3]).double()
+        t = T[i:, ..., :3].double()
+        err[i:] = (d * d).sum((1, 2, 3)) / ((t * t).sum((1, 2, 3)) + 1e-12)
+    return err
+
+
+def train(net, X, T, steps, batch=16, lr=1e-3, log=print, seed=0, log_every=50, schedule="cosine", hard=0.0,
+          refresh=5000):
+    """Adam on random crops.  hard > 0: every `refresh` steps the crops'
+    relative errors under the current network are measured, and a batch
+    draws a crop with probability (1 - hard) / n + hard * err / sum(err):
+    crops the network gets wrong relative to their own energy (dark scenes,
+    small lit regions) are seen more often than their share of the
+    absolute loss would give them."""
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
     hist = []
     t0 = time.time()
     torch.cuda.synchronize()
+    prob = None
     for it in range(steps):
-        idx = torch.randint(0, X.shape[0], (batch,), device="cuda", generator=gen)
+        if hard > 0 and it % refresh == 0:
+            err = crop_errors(net, X, T, batch)
+            prob = ((1 - hard) / X.shape[0] + hard * err / err.sum()).float()
+            log(f"step {it}: crop relative error mean {float(err.mean()):.5f}, max {float(err.max()):.5f}")
+        if prob is None:
+            idx = torch.randint(0, X.shape[0], (batch,), device="cuda", generator=gen)
+        else:
+            idx = torch.multinomial(prob, batch, replacement=True, generator=gen)
         xb = X.index_select(0, idx).contiguous()
         tb = T.index_select(0, idx).contiguous()
         loss.zero_()
@@ -198,6 +232,9 @@ def main(argv=None):
     t.add_argument("--loss", default="l1", choices=("l1", "l2"))
     t.add_argument("--grid", type=int, nargs="+", default=None, help="pattern width [height]")
     t.add_argument("--dataset-seed", type=int, default=1000, help="first scene seed of the training pairs")
+    t.add_argument("--recipe", default=None, help="text stored as the checkpoint's 'training' metadata")
+    t.add_argument("--hard", type=float, default=0.0,
+                   help="share of each batch drawn in proportion to the crops' current relative error")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--seeds", default="200,201,202,203,204,205,206,207")
@@ -219,9 +256,9 @@ def main(argv=None):
             net.load(args.resume)
         net.set_loss(args.loss)
         net.autotune(args.batch, CROP, CROP, backward=True)
-        hist, secs = train(net, X, T, args.steps, args.batch, args.lr, log=log, schedule=args.schedule)
+        hist, secs = train(net, X, T, args.steps, args.batch, args.lr, log=log, schedule=args.schedule, hard=args.hard)
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-        net.save(args.out)
+        net.save(args.out, metadata={"training": args.recipe} if args.recipe else None)
         log(f"saved {args.out}: {args.steps} steps in {secs:.1f} s ({args.steps / secs:.2f} steps/s)")
         del X, T
         infer = dev.UNet(1, H, W)
@@ -234,7 +271,7 @@ def main(argv=None):
                 json.dump({"history": hist, "train_seconds": secs, "steps": args.steps, "batch": args.batch,
                            "scenes": args.scenes, "poses": args.poses, "footprint": args.footprint,
                            "schedule": args.schedule, "loss": args.loss, "lr": args.lr, "grid": args.grid,
-                           "resume": args.resume, "dataset_seed": args.dataset_seed,
+                           "resume": args.resume, "dataset_seed": args.dataset_seed, "hard": args.hard,
                            "eval": rows}, f)
     else:
         args.grid = NOISY_GRID if args.grid is None else (args.grid[0] if len(args.grid) == 1 else tuple(args.grid))

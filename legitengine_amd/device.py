@@ -349,13 +349,15 @@ class UNet:
                   "lgx_unet_denoise")
         return out
 
-    def save(self, path):
-        """Checkpoint: parameters and Adam state (safetensors, no code executed on load)."""
+    def save(self, path, metadata=None):
+        """Checkpoint: parameters and Adam state (safetensors, no code executed
+        on load); metadata: optional dict of strings stored in the header."""
         from safetensors.numpy import save_file
         p, t = self.get(0)
         m, _ = self.get(2)
         v, _ = self.get(3)
-        save_file({"params": p, "adam_m": m, "adam_v": v, "adam_t": np.array([t], np.int32)}, path)
+        save_file({"params": p, "adam_m": m, "adam_v": v, "adam_t": np.array([t], np.int32)}, path,
+                  metadata=metadata)
 
     def load(self, path):
         from safetensors.numpy import load_file

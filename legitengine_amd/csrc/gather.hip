@@ -144,9 +144,9 @@ template <class T> LGX_D T pseudo_angle(T y, T x) {
   return y >= T(0) ? T(1) - r : r - T(1);
 }
 
-template <class T> LGX_D Vec3<T> unproject_far(const GatherArgs<T> &a, T sx, T sy) {
+template <class T, class ARGS> LGX_D Vec3<T> unproject_far(ARGS &a, T sx, T sy) {
   // indirectLighting.frag:20-25 with screenPos.z = 1 (:241)
-  const T *m = a.inv_vp;
+  const auto &m = a.inv_vp;
   T x = sx * T(2) - T(1), y = sy * T(2) - T(1);
   T vx = m[0] * x + m[4] * y + m[8] * T(1) + m[12] * T(1);
   T vy = m[1] * x + m[5] * y + m[9] * T(1) + m[13] * T(1);
@@ -180,8 +180,8 @@ LGX_D void bilinear(const void *base, int pitch, int lw, int lh, T sx, T sy, T *
   }
 }
 
-template <class T, int FMT>
-LGX_D void sample_trilinear(const void *const *lv, const int *pitch, const int *lw, const int *lh, int nl, T sx, T sy,
+template <class T, int FMT, class LV, class PI>
+LGX_D void sample_trilinear(const LV &lv, const PI &pitch, const PI &lw, const PI &lh, int nl, T sx, T sy,
                             T lod, T *out, int nch) {
   T q = T(nl - 1);
   T d = lod;
@@ -215,8 +215,11 @@ template <class T> LGX_D size_t point_index(T sx, T sy, int lw, int lh, int pitc
 // T: per-step arithmetic; A: accumulation of the slice contributions and of
 // the direction sum (float/float, double/double, or float/double).
 // One workgroup's share: logical block b (slice k = b % (grid grid_y), tile b / (grid grid_y)).
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT>
-LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
+// ARGS: GatherArgs<T> as the kernel's argument, or in constant memory
+// (address space 4, the multi-view kernel's per-view table): either way the
+// compiler reads its fields as scalars.
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, class ARGS>
+LGX_D void gather_block(ARGS &a, const uint32_t b) {
   const int g2 = a.grid * a.grid_y;
   const int k = (int)(b % (uint32_t)g2);
   const int t = (int)(b / (uint32_t)g2);
@@ -385,7 +388,7 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
           r = T(a.pd[l][point_index(sqx, sqy, a.lw[l], a.lh[l], a.pd_pitch[l])]);
         } else {
           T tmp[4];
-          sample_trilinear<T, LGX_R32F>((const void *const *)a.pd, a.pd_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod,
+          sample_trilinear<T, LGX_R32F>(a.pd, a.pd_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod,
                                         tmp, 1);
           r = tmp[0];
         }
@@ -417,7 +420,7 @@ LGX_D void gather_block(const GatherArgs<T> &a, const uint32_t b) {
             ls[1] = T(v.y);
             ls[2] = T(v.z);
           } else {
-            sample_trilinear<T, RFMT>((const void *const *)a.pr, a.pr_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod, ls,
+            sample_trilinear<T, RFMT>(a.pr, a.pr_pitch, a.lw, a.lh, a.n_levels, sqx, sqy, lod, ls,
                                       3);
           }
           L0 += A(ls[0] * hc);  // :261
@@ -482,6 +485,10 @@ __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs
                                                            const uint32_t *__restrict__ first, int n_views,
                                                            uint32_t *next) {
   __shared__ uint32_t s_item;
+  // the views' arguments are only read: constant memory, so their fields
+  // (with a uniform view index) become scalar loads, as a kernel argument's are
+  typedef const GatherArgs<T> __attribute__((address_space(4))) CArgs;
+  CArgs *cviews = (CArgs *)views;
   const uint32_t X = 8;
   const uint32_t total = first[n_views];
   const uint32_t xcd = blockIdx.x % X;
@@ -493,11 +500,11 @@ __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs
     for (;;) {
       if (threadIdx.x == 0) s_item = c0 + atomicAdd(&next[y], 1u);
       __syncthreads();
-      const uint32_t it = s_item;
+      const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
       __syncthreads();  // every wave has read s_item before lane 0 overwrites it
       if (it >= c1) break;
       while (v + 1 < n_views && it >= first[v + 1]) v++;  // items of one chunk only grow
-      gather_block<T, A, SAMPLER, RFMT, NFMT, false>(views[v], it - first[v]);
+      gather_block<T, A, SAMPLER, RFMT, NFMT, false>(cviews[v], it - first[v]);
     }
   }
 }

@@ -61,6 +61,21 @@ LGX_D bf16x8 load8(const ConvInput &in, int n, int y, int x, int h, int w, int c
   return *(const bf16x8 *)(in.b + (((size_t)n * h + y) * w + x) * in.cb + (c - in.ca));
 }
 
+// The same 16 bytes for a pixel clamped into the image (always a valid
+// read; the caller zeroes what lies outside).
+LGX_D bf16x8 load8_clamped(const ConvInput &in, int n, int y, int x, int h, int w, int c) {
+  y = min(max(y, 0), h - 1);
+  x = min(max(x, 0), w - 1);
+  if (c < in.ca) {
+    if (in.a_up) {
+      const int hh = h >> 1, ww = w >> 1;
+      return *(const bf16x8 *)(in.a + (((size_t)n * hh + (y >> 1)) * ww + (x >> 1)) * in.ca + c);
+    }
+    return *(const bf16x8 *)(in.a + (((size_t)n * h + y) * w + x) * in.ca + c);
+  }
+  return *(const bf16x8 *)(in.b + (((size_t)n * h + y) * w + x) * in.cb + (c - in.ca));
+}
+
 // Stores one output row of a wave's register tile: PB blocks of 16 pixels
 // (x0 + pb * 16 + col) x MB blocks of 16 channels (co0 + mb * 16 + kg * 4,
 // 4 per lane): bias (bias_r, this lane's channels), ReLU, the optional
@@ -625,36 +640,72 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
   auto pair = [](bf16 lo, bf16 hi) {
     return (uint32_t)__builtin_bit_cast(u16, lo) | ((uint32_t)__builtin_bit_cast(u16, hi) << 16);
   };
+  // The next round's rows are loaded into registers while the current
+  // round's matrix work runs (loads unconditional, from clamped in-image
+  // addresses; what lies outside the image or the round is staged as zero).
+  constexpr int XI = ((RB + 2) * (XS / 2) * 4 + 255) / 256;  // input items per thread
+  constexpr int DI = (RB * 32 * (COUT / 8) + 255) / 256;    // output-gradient items per thread
+  bf16x8 px0[XI], px1[XI], pd0[DI], pd1[DI];
+  uint32_t okx = 0, okd = 0;  // bit 2i / 2i+1: item i's first / second pixel lies inside
+  auto fetch = [&](int yr, int nr) {
+    okx = okd = 0;
+#pragma unroll
+    for (int i = 0; i < XI; i++) {
+      const int e = min((int)threadIdx.x + i * 256, (RB + 2) * (XS / 2) * 4 - 1);
+      const int q = e & 3, pp = (e >> 2) % (XS / 2), rr = (e >> 2) / (XS / 2);
+      const int y = yr + rr - 1, x = x0 - 8 + pp * 2;
+      const bool rowok = rr < nr + 2 && y >= 0 && y < a.h;
+      px0[i] = load8_clamped(in, n, y, x, a.h, a.w, g * 32 + q * 8);
+      px1[i] = load8_clamped(in, n, y, x + 1, a.h, a.w, g * 32 + q * 8);
+      okx |= (uint32_t)(rowok && x >= 0 && x < a.w) << (2 * i);
+      okx |= (uint32_t)(rowok && x + 1 >= 0 && x + 1 < a.w) << (2 * i + 1);
+    }
+#pragma unroll
+    for (int i = 0; i < DI; i++) {
+      const int e = min((int)threadIdx.x + i * 256, RB * 32 * (COUT / 8) - 1);
+      const int q = e % (COUT / 8), pp = (e / (COUT / 8)) % 32, rr = e / (COUT / 8) / 32;
+      const int y = min(yr + rr, a.h - 1), x = x0 + pp * 2;
+      const bf16 *row = a.dy + ((size_t)n * a.h + y) * a.w * COUT + q * 8;
+      pd0[i] = *(const bf16x8 *)(row + (size_t)min(x, a.w - 1) * COUT);
+      pd1[i] = *(const bf16x8 *)(row + (size_t)min(x + 1, a.w - 1) * COUT);
+      okd |= (uint32_t)(rr < nr && x < a.w) << (2 * i);
+      okd |= (uint32_t)(rr < nr && x + 1 < a.w) << (2 * i + 1);
+    }
+  };
+  if (y0 < a.h) fetch(y0, min(RB, min(a.rows, a.h - y0)));
   for (int r0 = 0; r0 < a.rows; r0 += RB) {
     const int yr = y0 + r0;
     if (yr >= a.h) break;  // uniform across the workgroup
     const int nr = min(RB, min(a.rows - r0, a.h - yr));
     // input rows yr - 1 .. yr + nr (32 channels of group g), transposed, pixel pairs
-    for (int e = threadIdx.x; e < (RB + 2) * (XS / 2) * 4; e += 256) {
-      const int q = e & 3, pp = (e >> 2) % (XS / 2), rr = (e >> 2) / (XS / 2);
-      const int px = pp * 2;
-      bf16x8 v0 = {}, v1 = {};
-      if (rr < nr + 2) {
-        v0 = load8(in, n, yr + rr - 1, x0 - 8 + px, a.h, a.w, g * 32 + q * 8);
-        v1 = load8(in, n, yr + rr - 1, x0 - 8 + px + 1, a.h, a.w, g * 32 + q * 8);
-      }
 #pragma unroll
-      for (int i = 0; i < 8; i++) *(uint32_t *)&xs[rr][q * 8 + i][px] = pair(v0[i], v1[i]);
+    for (int i = 0; i < XI; i++) {
+      const int e = (int)threadIdx.x + i * 256;
+      if (e < (RB + 2) * (XS / 2) * 4) {
+        const int q = e & 3, pp = (e >> 2) % (XS / 2), rr = (e >> 2) / (XS / 2);
+        const bf16x8 z = {};
+        const bf16x8 v0 = (okx >> (2 * i)) & 1u ? px0[i] : z, v1 = (okx >> (2 * i + 1)) & 1u ? px1[i] : z;
+#pragma unroll
+        for (int k = 0; k < 8; k++) *(uint32_t *)&xs[rr][q * 8 + k][pp * 2] = pair(v0[k], v1[k]);
+      }
     }
     // output-gradient rows yr .. yr + nr - 1, transposed, pixel pairs
-    for (int e = threadIdx.x; e < RB * 32 * (COUT / 8); e += 256) {
-      const int q = e % (COUT / 8), pp = (e / (COUT / 8)) % 32, rr = e / (COUT / 8) / 32;
-      const int x = x0 + pp * 2;
-      bf16x8 v0 = {}, v1 = {};
-      if (rr < nr) {
-        const bf16 *row = a.dy + ((size_t)n * a.h + yr + rr) * a.w * COUT + q * 8;
-        if (x < a.w) v0 = *(const bf16x8 *)(row + (size_t)x * COUT);
-        if (x + 1 < a.w) v1 = *(const bf16x8 *)(row + (size_t)(x + 1) * COUT);
-      }
 #pragma unroll
-      for (int i = 0; i < 8; i++) *(uint32_t *)&ds[rr][q * 8 + i][pp * 2] = pair(v0[i], v1[i]);
+    for (int i = 0; i < DI; i++) {
+      const int e = (int)threadIdx.x + i * 256;
+      if (e < RB * 32 * (COUT / 8)) {
+        const int q = e % (COUT / 8), pp = (e / (COUT / 8)) % 32, rr = e / (COUT / 8) / 32;
+        const bf16x8 z = {};
+        const bf16x8 v0 = (okd >> (2 * i)) & 1u ? pd0[i] : z, v1 = (okd >> (2 * i + 1)) & 1u ? pd1[i] : z;
+#pragma unroll
+        for (int k = 0; k < 8; k++) *(uint32_t *)&ds[rr][q * 8 + k][pp * 2] = pair(v0[k], v1[k]);
+      }
     }
     __syncthreads();
+    {  // the next round's rows, in flight during this round's matrix work
+      const int yn = yr + RB;
+      if (r0 + RB < a.rows && yn < a.h) fetch(yn, min(RB, min(a.rows - r0 - RB, a.h - yn)));
+    }
     if (g == 0) {  // uniform across the workgroup
       for (int rr = 0; rr < nr; rr++)
 #pragma unroll

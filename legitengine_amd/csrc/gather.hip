@@ -59,7 +59,8 @@ template <class T> struct GatherArgs {
   int *step_counts;
   int *hit_counts;  // COUNT only, may be null
   int tile_w, tile_h, tiles_x, n_blocks;
-  int occ;  // host side: which register allocation to launch
+  int occ;   // host side: which register allocation to launch
+  int pipe;  // host side: which step loop to launch
   int sx0, sy0, lcells_x, lcells_y;  // slice cells covered by the launch (region of interest)
   int rx0, ry0, rx1, ry1;            // pixels gathered: [rx0, rx1) x [ry0, ry1)
 };
@@ -115,24 +116,34 @@ template <> LGX_D float atan2_finite<float>(float y, float x) {
 }
 
 // Radiance texel through a buffer resource (32-bit byte offset against a
-// scalar base), widened to float.
+// scalar base): the raw load, and its widening to float, so a fetch can be
+// issued in one step of the march and consumed in the next.
 template <int FMT> struct RadBuf;
 template <> struct RadBuf<LGX_RGBA16F> {
   static constexpr int kBytes = 8;
-  static LGX_D float4 load(__amdgpu_buffer_rsrc_t r, uint32_t ti) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(ti * 8u), 0, 0);
+  typedef unsigned int Raw __attribute__((ext_vector_type(2)));
+  static LGX_D Raw raw(__amdgpu_buffer_rsrc_t r, uint32_t ti) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, (int)(ti * 8u), 0, 0);
+  }
+  static LGX_D float4 load(__amdgpu_buffer_rsrc_t r, uint32_t ti) { return decode(raw(r, ti)); }
+  static LGX_D float4 decode(Raw v) {
     return make_float4(h2f((uint16_t)(v[0] & 0xffffu)), h2f((uint16_t)(v[0] >> 16)), h2f((uint16_t)(v[1] & 0xffffu)),
                        h2f((uint16_t)(v[1] >> 16)));
   }
 };
 template <> struct RadBuf<LGX_RGBA32F> {
   static constexpr int kBytes = 16;
-  static LGX_D float4 load(__amdgpu_buffer_rsrc_t r, uint32_t ti) {
+  typedef unsigned int Raw __attribute__((ext_vector_type(4)));
+  static LGX_D Raw raw(__amdgpu_buffer_rsrc_t r, uint32_t ti) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)(ti * 16u), 0, 0);
+  }
+  static LGX_D float4 load(__amdgpu_buffer_rsrc_t r, uint32_t ti) { return decode(raw(r, ti)); }
+  static LGX_D float4 decode(Raw v) {
     // the whole vector is reinterpreted at once: a bit cast of one element
-    // of the loaded vector (v[i]) compiles to a read of element 0 here
+    // of a loaded vector (v[i]) compiles to a read of element 0 here
     typedef float f32x4 __attribute__((ext_vector_type(4)));
-    const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(ti * 16u), 0, 0));
-    return make_float4(v.x, v.y, v.z, v.w);
+    const f32x4 f = __builtin_bit_cast(f32x4, v);
+    return make_float4(f.x, f.y, f.z, f.w);
   }
 };
 
@@ -218,7 +229,7 @@ template <class T> LGX_D size_t point_index(T sx, T sy, int lw, int lh, int pitc
 // ARGS: GatherArgs<T> as the kernel's argument, or in constant memory
 // (address space 4, the multi-view kernel's per-view table): either way the
 // compiler reads its fields as scalars.
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, class ARGS>
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, bool PIPE, class ARGS>
 LGX_D void gather_block(ARGS &a, const uint32_t b) {
   const int g2 = a.grid * a.grid_y;
   const int k = (int)(b % (uint32_t)g2);
@@ -295,6 +306,17 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
       // it is read as a scalar; the Hi-Z level floor(lod) never decreases
       // along a ray, so the level changes at table-given steps and its base,
       // pitch and extent stay in scalar registers for its run of steps.
+      // Two step loops with the same arithmetic in the same order (so the
+      // same results).  PIPE: fetches overlap arithmetic -- a step's depth
+      // texel is requested while the previous step's hit (angle, weights,
+      // accumulation) is computed, and a hit's radiance texel is requested
+      // before that work; the depth request of the step after a lane's last
+      // one is discarded (its offset may lie outside the level, for a ray
+      // leaving the screen, where the buffer resource's range check returns
+      // 0 instead of reading).  It hides more latency for a few more
+      // instructions per step, which pays on long marches (few directions, a
+      // wide footprint) and not on short ones; the launch-shape autotuner
+      // picks per configuration.  Otherwise each step fetches and then waits.
       const int *lstart = a.level_start + (size_t)k * (LGX_MAX_LEVELS + 1);
       int l = 0;
       int lnext = a.n_levels > 1 ? lstart[1] : INT_MAX;  // first step of the next level
@@ -311,13 +333,35 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
       __amdgpu_buffer_rsrc_t drs = depth_rsrc(0), rrs = rad_rsrc(0);
       T flw = T(a.lw[0]), flh = T(a.lh[0]);
       int lw1 = a.lw[0] - 1, lh1 = a.lh[0] - 1, dpitch = a.pd_pitch[0], rpitch = a.pr_pitch[0];
-      T off = steps[0];
-      for (;; j++) {
-        if (j >= iters) break;
-        const int ju = __builtin_amdgcn_readfirstlane(j);
-        // the next step's offset: a scalar load in flight during this step
-        const T off_next = steps[2 * min(ju + 1, a.S_tab - 1)];
-        while (ju >= lnext) {  // uniform: step ju lies on a later level
+      // the fetch of step s at offset o on the current level: on-screen flag,
+      // q / extent, depth request, radiance texel index
+      struct Fetch {
+        float rd;
+        T sqx, sqy;
+        uint32_t rti;
+        bool on;
+      };
+      auto request = [&](int s, T o) {
+        Fetch f;
+        const T qx = cx + dx * o, qy = cy + dy * o;  // :218
+        // :219-228 on screen: q / extent in (0, 1) is q in (0, extent), as
+        // the quotient is correctly rounded (it reaches 1 only from q = extent)
+        // (bitwise &: one mask, not a chain of divergent branches)
+        f.on = (s < iters) & (qx > T(0)) & (qx < a.W) & (qy > T(0)) & (qy < a.H);
+        f.sqx = div_extent(qx, a.W, a.rW);  // :219
+        f.sqy = div_extent(qy, a.H, a.rH);
+        // sq > 0 on screen, so sq * extent >= 0 and truncation is floor; the
+        // product can round up to the extent itself, hence the upper clamp
+        const int ix = min((int)(f.sqx * flw), lw1);
+        const int iy = min((int)(f.sqy * flh), lh1);
+        f.rti = (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix;
+        const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
+        f.rd = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0));
+        return f;
+      };
+      // the level of step s (uniform; s never decreases)
+      auto to_level = [&](int s) {
+        while (s >= lnext) {
           l++;
           lnext = l + 1 < a.n_levels ? lstart[l + 1] : INT_MAX;
           drs = depth_rsrc(l);
@@ -329,52 +373,113 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
           lw1 = a.lw[l] - 1;
           lh1 = a.lh[l] - 1;
         }
-        const T qx = cx + dx * off, qy = cy + dy * off;  // :218
-        // :219-228 on screen: q / extent in (0, 1) is q in (0, extent), as
-        // the quotient is correctly rounded (it reaches 1 only from q = extent)
-        // (bitwise &: one mask, not a chain of divergent branches)
-        if (!((qx > T(0)) & (qx < a.W) & (qy > T(0)) & (qy < a.H))) break;
-        const T sqx = div_extent(qx, a.W, a.rW), sqy = div_extent(qy, a.H, a.rH);  // :219
-        // sq > 0, so sq * extent >= 0 and truncation is floor; the product can
-        // round up to the extent itself, hence the upper clamp (in integers)
-        const int ix = min((int)(sqx * flw), lw1);
-        const int iy = min((int)(sqy * flh), lh1);
-        const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
-        const T r = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0)));
-        // :241-252 sample delta projected on (eye, tangent)
-        const T len2 = fma_(off, fma_(off, vv, uv2), uu);
-        const T ri = r * fast_rsqrt(len2);
-        const T hyv = ri * fma_(off, tv, tu) - trc;
-        const T hxv = ri * fma_(off, ev, eu) - erc;
-        const T pa = pseudo_angle(hyv, hxv);
-        if (pa < pa_max) {  // :254
-          if (COUNT) hit_total++;
-          const T th = atan2_finite(hyv, hxv);
-          const T iq = fast_rcp(hxv * hxv + hyv * hyv);
-          const T c2 = (hxv * hxv - hyv * hyv) * iq, s2 = (T(2) * hxv * hyv) * iq;
-          T side = T(1);
-          {
-            const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
-            side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
-            side *= min(max(sqx * inv_width, T(0)), T(1));
-            side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
-            side *= min(max(sqy * inv_width, T(0)), T(1));
+      };
+      if constexpr (PIPE) {
+        T off = steps[0];
+        T off_n = steps[2 * min(1, a.S_tab - 1)];  // the next step's offset (scalar)
+        to_level(0);
+        Fetch f = request(0, off);
+        for (; f.on; j++) {
+          const int ju = __builtin_amdgcn_readfirstlane(j);
+          // step j: :241-252 sample delta projected on (eye, tangent)
+          const T r = T(f.rd);
+          const T len2 = fma_(off, fma_(off, vv, uv2), uu);
+          const T ri = r * fast_rsqrt(len2);
+          const T hyv = ri * fma_(off, tv, tu) - trc;
+          const T hxv = ri * fma_(off, ev, eu) - erc;
+          const T pa = pseudo_angle(hyv, hxv);
+          const bool hit = pa < pa_max;  // :254
+          typename RadBuf<RFMT>::Raw rv;
+          if (hit) rv = RadBuf<RFMT>::raw(rrs, f.rti);  // :256, on step j's level
+          const T sqx = f.sqx, sqy = f.sqy;
+          // step j + 1: its level, then its depth request
+          off = off_n;
+          off_n = steps[2 * min(ju + 2, a.S_tab - 1)];  // scalar load, in flight for a step
+          to_level(ju + 1);
+          f = request(ju + 1, off);
+          if (hit) {
+            if (COUNT) hit_total++;
+            const T th = atan2_finite(hyv, hxv);
+            const T iq = fast_rcp(hxv * hxv + hyv * hyv);
+            const T c2 = (hxv * hxv - hyv * hyv) * iq, s2 = (T(2) * hxv * hyv) * iq;
+            T side = T(1);
+            {
+              const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
+              side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
+              side *= min(max(sqx * inv_width, T(0)), T(1));
+              side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
+              side *= min(max(sqy * inv_width, T(0)), T(1));
+            }
+            const T hc =
+                (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
+            const float4 v = RadBuf<RFMT>::decode(rv);
+            L0 += A(T(v.x) * hc);  // :261
+            L1 += A(T(v.y) * hc);
+            L2 += A(T(v.z) * hc);
+            L0 -= A(ambient * hc);  // :262
+            L1 -= A(ambient * hc);
+            L2 -= A(ambient * hc);
+            max_h = th;  // :263
+            pa_max = pa;
+            c2max = c2;
+            s2max = s2;
           }
-          const T hc =
-              (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
-          const float4 v = RadBuf<RFMT>::load(rrs, (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix);  // :256
-          L0 += A(T(v.x) * hc);  // :261
-          L1 += A(T(v.y) * hc);
-          L2 += A(T(v.z) * hc);
-          L0 -= A(ambient * hc);  // :262
-          L1 -= A(ambient * hc);
-          L2 -= A(ambient * hc);
-          max_h = th;  // :263
-          pa_max = pa;
-          c2max = c2;
-          s2max = s2;
         }
-        off = off_next;
+      } else {
+        T off = steps[0];
+        for (;; j++) {
+          if (j >= iters) break;
+          const int ju = __builtin_amdgcn_readfirstlane(j);
+          // the next step's offset: a scalar load in flight during this step
+          const T off_next = steps[2 * min(ju + 1, a.S_tab - 1)];
+          to_level(ju);
+          const T qx = cx + dx * off, qy = cy + dy * off;  // :218
+          // :219-228 on screen: q / extent in (0, 1) is q in (0, extent), as
+          // the quotient is correctly rounded (it reaches 1 only from q = extent)
+          // (bitwise &: one mask, not a chain of divergent branches)
+          if (!((qx > T(0)) & (qx < a.W) & (qy > T(0)) & (qy < a.H))) break;
+          const T sqx = div_extent(qx, a.W, a.rW), sqy = div_extent(qy, a.H, a.rH);  // :219
+          // sq > 0, so sq * extent >= 0 and truncation is floor; the product can
+          // round up to the extent itself, hence the upper clamp (in integers)
+          const int ix = min((int)(sqx * flw), lw1);
+          const int iy = min((int)(sqy * flh), lh1);
+          const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
+          const T r = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0)));
+          // :241-252 sample delta projected on (eye, tangent)
+          const T len2 = fma_(off, fma_(off, vv, uv2), uu);
+          const T ri = r * fast_rsqrt(len2);
+          const T hyv = ri * fma_(off, tv, tu) - trc;
+          const T hxv = ri * fma_(off, ev, eu) - erc;
+          const T pa = pseudo_angle(hyv, hxv);
+          if (pa < pa_max) {  // :254
+            if (COUNT) hit_total++;
+            const T th = atan2_finite(hyv, hxv);
+            const T iq = fast_rcp(hxv * hxv + hyv * hyv);
+            const T c2 = (hxv * hxv - hyv * hyv) * iq, s2 = (T(2) * hxv * hyv) * iq;
+            T side = T(1);
+            {
+              const T width = T(0.1f), inv_width = T(1) / width;  // :222-227
+              side *= min(max((T(1) - sqx) * inv_width, T(0)), T(1));
+              side *= min(max(sqx * inv_width, T(0)), T(1));
+              side *= min(max((T(1) - sqy) * inv_width, T(0)), T(1));
+              side *= min(max(sqy * inv_width, T(0)), T(1));
+            }
+            const T hc =
+                (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
+            const float4 v = RadBuf<RFMT>::load(rrs, (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix);  // :256
+            L0 += A(T(v.x) * hc);  // :261
+            L1 += A(T(v.y) * hc);
+            L2 += A(T(v.z) * hc);
+            L0 -= A(ambient * hc);  // :262
+            L1 -= A(ambient * hc);
+            L2 -= A(ambient * hc);
+            max_h = th;  // :263
+            pa_max = pa;
+            c2max = c2;
+            s2max = s2;
+          }
+          off = off_next;
+        }
       }
     } else {
       for (; j < iters; j++) {
@@ -464,10 +569,11 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
 }
 
 // OCC: minimum waves per SIMD the register allocation must allow (1 = the
-// compiler's choice); the launch shapes offer 1 and 8.
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, int OCC>
+// compiler's choice); the launch shapes offer 1 and 8.  PIPE: the step loop
+// that overlaps fetches with arithmetic (Hi-Z sampler).
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, int OCC, bool PIPE = false>
 __global__ __launch_bounds__(256, OCC) void gather_kernel(const GatherArgs<T> a) {
-  gather_block<T, A, SAMPLER, RFMT, NFMT, COUNT>(a, xcd_remap(blockIdx.x, (uint32_t)a.n_blocks));
+  gather_block<T, A, SAMPLER, RFMT, NFMT, COUNT, PIPE>(a, xcd_remap(blockIdx.x, (uint32_t)a.n_blocks));
 }
 
 // Persistent gather over several views: the logical blocks of all views form
@@ -480,7 +586,7 @@ __global__ __launch_bounds__(256, OCC) void gather_kernel(const GatherArgs<T> a)
 // workgroups idle at the tail.  A workgroup whose own chunk is exhausted
 // takes blocks from the other XCDs' chunks in turn (the counters only grow,
 // so every block is taken once), and ends when all eight are exhausted.
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, int OCC>
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, int OCC, bool PIPE>
 __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs<T> *__restrict__ views,
                                                            const uint32_t *__restrict__ first, int n_views,
                                                            uint32_t *next) {
@@ -504,7 +610,7 @@ __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs
       __syncthreads();  // every wave has read s_item before lane 0 overwrites it
       if (it >= c1) break;
       while (v + 1 < n_views && it >= first[v + 1]) v++;  // items of one chunk only grow
-      gather_block<T, A, SAMPLER, RFMT, NFMT, false>(cviews[v], it - first[v]);
+      gather_block<T, A, SAMPLER, RFMT, NFMT, false, PIPE>(cviews[v], it - first[v]);
     }
   }
 }
@@ -692,14 +798,17 @@ template <class T> static void frame_constants(const lgx_gather_params *p, Gathe
 template <class T, class A, int SAMPLER, int RFMT, int NFMT>
 static hipError_t launch_gather(const GatherArgs<T> &a, bool count, hipStream_t s) {
   dim3 grid((unsigned)a.n_blocks), block(256);
+  // the FP32 Hi-Z gather has every variant; the others run the default one
+  constexpr bool kAll = sizeof(T) == 4 && sizeof(A) == 4 && SAMPLER == LGX_SAMPLER_HIZ_POINT;
   if (count)
     hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, true, 1>), grid, block, 0, s, a);
-  else if (a.occ == 8) {
-    if constexpr (sizeof(T) == 4 && sizeof(A) == 4)
-      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 8>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1>), grid, block, 0, s, a);
-  } else
+  else if (kAll && a.occ == 8 && a.pipe)
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, kAll ? 8 : 1, kAll>), grid, block, 0, s, a);
+  else if (kAll && a.occ == 8)
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, kAll ? 8 : 1>), grid, block, 0, s, a);
+  else if (kAll && a.pipe)
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1, kAll>), grid, block, 0, s, a);
+  else
     hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1>), grid, block, 0, s, a);
   return hipGetLastError();
 }
@@ -714,12 +823,15 @@ static hipError_t dispatch_fmt(const GatherArgs<T> &a, int rfmt, int nfmt, bool 
 }
 
 // Launch shapes offered to the autotuner: tile_w x tile_h slice pixels per
-// 256-thread workgroup, and the occupancy the registers are allocated for
-// (1: the compiler's choice, 79 VGPRs, 6 waves per SIMD; 8: 64 VGPRs with a
-// few spilled set-up values).  Every shape computes the same arithmetic.
-static const int kTiles[][3] = {{16, 16, 1}, {8, 32, 1}, {32, 8, 1}, {64, 4, 1},
-                                {16, 16, 8}, {8, 32, 8}, {32, 8, 8}, {64, 4, 8}};
-static const int kNumTiles = 8;
+// 256-thread workgroup, the occupancy the registers are allocated for (1: the
+// compiler's choice, 6 waves per SIMD; 8: 64 VGPRs with a few spilled set-up
+// values), and the step loop (1: fetches overlapped with arithmetic).  Every
+// shape computes the same arithmetic in the same order.
+static const int kTiles[][4] = {{16, 16, 1, 0}, {8, 32, 1, 0}, {32, 8, 1, 0}, {64, 4, 1, 0},
+                                {16, 16, 8, 0}, {8, 32, 8, 0}, {32, 8, 8, 0}, {64, 4, 8, 0},
+                                {16, 16, 1, 1}, {8, 32, 1, 1}, {32, 8, 1, 1}, {64, 4, 1, 1},
+                                {16, 16, 8, 1}, {8, 32, 8, 1}, {32, 8, 8, 1}, {64, 4, 8, 1}};
+static const int kNumTiles = 16;
 
 }  // namespace lgx
 
@@ -794,6 +906,7 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
   a.tile_w = kTiles[tile][0];
   a.tile_h = kTiles[tile][1];
   a.occ = kTiles[tile][2];
+  a.pipe = kTiles[tile][3];
   a.rx0 = 0;
   a.ry0 = 0;
   a.rx1 = p->width;
@@ -926,19 +1039,25 @@ static int gather_batch_upload(lgx_gather_batch *b, int tile) {
 
 template <int SAMPLER, int RFMT, int NFMT>
 static hipError_t launch_multi(const lgx_gather_batch *b, unsigned grid, hipStream_t s) {
-  const bool occ8 = kTiles[b->tile][2] == 8;
+  const bool occ8 = kTiles[b->tile][2] == 8, pipe = kTiles[b->tile][3] == 1;
   const int n = (int)b->p.size();
   const GatherArgs<float> *args = (const GatherArgs<float> *)b->dev;
   const uint32_t *first = (const uint32_t *)((const unsigned char *)b->dev + (size_t)n * sizeof(GatherArgs<float>));
   uint32_t *next = (uint32_t *)(first + n + 1);
   // a kernel, not a memset, so the launch can be captured into a graph
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, next, 8);
-  if (occ8)
-    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8>), dim3(grid), dim3(256), 0, s, args,
-                       first, n, next);
+  if (occ8 && pipe)
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, true>), dim3(grid), dim3(256), 0, s,
+                       args, first, n, next);
+  else if (occ8)
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, false>), dim3(grid), dim3(256), 0, s,
+                       args, first, n, next);
+  else if (pipe)
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, true>), dim3(grid), dim3(256), 0, s,
+                       args, first, n, next);
   else
-    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1>), dim3(grid), dim3(256), 0, s, args,
-                       first, n, next);
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, false>), dim3(grid), dim3(256), 0, s,
+                       args, first, n, next);
   return hipGetLastError();
 }
 

@@ -227,7 +227,7 @@ def test_gather_slices_layout(dev, c1):
     assert back.numpy().tobytes() == a.numpy().tobytes()
 
 
-@pytest.mark.parametrize("tile", list(range(8)))  # every launch shape lgx_gather_tiles() offers
+@pytest.mark.parametrize("tile", list(range(16)))  # every launch shape lgx_gather_tiles() offers
 def test_gather_tiles_identical(dev, c1, tile):
     sc, view, proj, gb = c1[1]
     g = dev.GBufferDev.from_host(gb)

@@ -42,7 +42,7 @@ template <class T> struct GatherArgs {
   T W, H, near_step, log_ratio, ambient;
   int w, h, grid, grid_y, cells_x, cells_y, D, S_tab;  // pattern grid x grid_y
   T rW, rH;        // RN(1/W), RN(1/H) for the exact division by the extent
-  const T *dirs;   // [grid * grid_y][D][2]
+  const T *dirs;   // [grid * grid_y][D][4] = (cos, sin, 1 / cos, 1 / sin)
   const T *steps;  // [grid * grid_y][S_tab][2] = (offset, lod)
   const int *level_start;  // [grid * grid_y][LGX_MAX_LEVELS + 1]: first step at or above each Hi-Z level
   const float *depth;
@@ -260,30 +260,32 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
   const T erc = rc * dot3(eye, rayc);
   const T en = dot3(eye, n);
 
-  const T *dirs = a.dirs + (size_t)k * a.D * 2;
+  const T *dirs = a.dirs + (size_t)k * a.D * 4;
   const T *steps = a.steps + (size_t)k * a.S_tab * 2;
   A sum0 = 0, sum1 = 0, sum2 = 0;
   int step_total = 0, hit_total = 0;
 
   for (int d = 0; d < a.D; d++) {
-    const T dx = dirs[2 * d], dy = dirs[2 * d + 1];
-    // :180-183 tangent of the slice through the offset point
-    const T ox = (cx + dx * T(10e-1)) / a.W, oy = (cy + dy * T(10e-1)) / a.H;
+    // the direction and its reciprocals (the host's correctly rounded 1 / d)
+    const T dx = dirs[4 * d], dy = dirs[4 * d + 1], idx = dirs[4 * d + 2], idy = dirs[4 * d + 3];
+    // :180-183 tangent of the slice through the offset point (the quotient
+    // by the extent is the IEEE one, as for the step positions)
+    const T ox = div_extent(cx + dx * T(10e-1), a.W, a.rW), oy = div_extent(cy + dy * T(10e-1), a.H, a.rH);
     const Vec3<T> offp = cam + normalize3(unproject_far(a, ox, oy) - cam) * rc;
     const Vec3<T> tangent = normalize3(normalize3(offp - cam) - pdir);
     // :193-198 normal-limited initial horizon
     const Vec3<T> c1 = cross3(tangent, eye);
     const Vec3<T> dn = cross3(Vec3<T>{-c1.x, -c1.y, -c1.z}, n);
     const T hy = dot3(dn, tangent), hx = dot3(dn, eye);
-    T max_h = atan2(hy, hx);
+    T max_h = atan2_finite(hy, hx);
     T pa_max = pseudo_angle(hy, hx);
-    T q2 = hx * hx + hy * hy;
-    T c2max = (hx * hx - hy * hy) / q2, s2max = (T(2) * hx * hy) / q2;
+    const T iq2 = fast_rcp(hx * hx + hy * hy);
+    T c2max = (hx * hx - hy * hy) * iq2, s2max = (T(2) * hx * hy) * iq2;
     const T tn = dot3(tangent, n);
     // :202-204 path to the screen border
     T path;
     {
-      T ix = T(1) / dx, iy = T(1) / dy;
+      const T ix = idx, iy = idy;
       T t1 = (T(0) - cx) * ix, t2 = (a.W - cx) * ix, t3 = (T(0) - cy) * iy, t4 = (a.H - cy) * iy;
       path = fabs(min(max(t1, t2), max(t3, t4)));
     }
@@ -721,7 +723,7 @@ static bool fill_tables(const lgx_gather_params *p, const GatherTableShape &s, s
   const T W = T(p->width);
   const T near_step = W / T(1000);
   const int D = p->dirs;
-  const size_t nd = (size_t)s.g2 * D * 2, ns = (size_t)s.g2 * s.S_tab * 2;
+  const size_t nd = (size_t)s.g2 * D * 4, ns = (size_t)s.g2 * s.S_tab * 2;
   out.assign((nd + ns) * sizeof(T) + (size_t)s.g2 * (LGX_MAX_LEVELS + 1) * sizeof(int32_t), 0);
   T *dirs = (T *)out.data();
   T *steps = dirs + nd;
@@ -733,8 +735,12 @@ static bool fill_tables(const lgx_gather_params *p, const GatherTableShape &s, s
     const T pixel_ang_offset = T(2) * pi / T(D) * ang_offset;
     for (int d = 0; d < D; d++) {
       T screen_ang = pixel_ang_offset + T(2) * pi / T(D) * T(d);
-      dirs[((size_t)k * D + d) * 2 + 0] = std::cos(screen_ang);
-      dirs[((size_t)k * D + d) * 2 + 1] = std::sin(screen_ang);
+      const T c = std::cos(screen_ang), sn = std::sin(screen_ang);
+      T *e = dirs + ((size_t)k * D + d) * 4;
+      e[0] = c;
+      e[1] = sn;
+      e[2] = T(1) / c;  // the march's box test divides by the direction (:202)
+      e[3] = T(1) / sn;
     }
     const T Df = T(footprint(p));
     const T ratio = T(2) * pi / Df + T(1);
@@ -840,7 +846,7 @@ using namespace lgx;
 extern "C" size_t lgx_gather_workspace_bytes(const lgx_gather_params *p) {
   if (!gather_params_ok(p)) return 0;
   GatherTableShape s = table_shape(p);
-  return ((size_t)s.g2 * s.D * 2 + (size_t)s.g2 * s.S_tab * 2) * s.elem +
+  return ((size_t)s.g2 * s.D * 4 + (size_t)s.g2 * s.S_tab * 2) * s.elem +
          (size_t)s.g2 * (LGX_MAX_LEVELS + 1) * sizeof(int32_t);
 }
 
@@ -879,7 +885,7 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
   a.D = p->dirs;
   a.S_tab = s.S_tab;
   a.dirs = (const T *)ws;
-  a.steps = (const T *)ws + (size_t)s.g2 * s.D * 2;
+  a.steps = (const T *)ws + (size_t)s.g2 * s.D * 4;
   a.level_start = (const int *)(a.steps + (size_t)s.g2 * s.S_tab * 2);
   a.rW = T(1) / a.W;
   a.rH = T(1) / a.H;

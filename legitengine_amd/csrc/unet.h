@@ -56,7 +56,8 @@ struct WgradArgs {
   int cout;
   int n, h, w;
   int rows;        // rows per workgroup (set by conv3x3_wgrad)
-  float *dw;       // [9][ca+cb][cout], overwritten
+  int cin_log;     // input channels of the master weights (the rest of ca + cb is zero padding)
+  float *dw;       // [cout][cin_log][9] (master layout), overwritten
   float *db;       // [cout] bias gradient, overwritten
   float *part;     // scratch for the per-workgroup partial sums
   float *part_b;   // (set by conv3x3_wgrad) bias partials inside part
@@ -81,11 +82,23 @@ hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const flo
                     int out_pitch, float4 *out, hipStream_t s);
 hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
                     float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s);
-hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
-                float bc1, float bc2, hipStream_t s);
+// The convolutions' weights inside the flat parameter vector and their BF16
+// packed copies (pack_weights' layouts), for adam_pack.
+struct PackLayer {
+  size_t off;  // first weight [co][ci][tap] in the parameter vector
+  int cout, cin_log, cin_pad;
+  bf16 *wf, *wt;  // wt may be null
+};
+struct PackTable {
+  PackLayer l[10];
+  int n;
+};
+// Adam (bias-corrected) on the FP32 parameters; the updated convolution
+// weights are also written to their packed BF16 copies.
+hipError_t adam_pack(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                     float bc1, float bc2, const PackTable &pk, hipStream_t s);
 hipError_t fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t pack_weights(const float *w, int cout, int cin_log, int cin_pad, bf16 *wf, bf16 *wt, hipStream_t s);
-hipError_t unpack_wgrad(const float *dw, int cout, int cin_log, int cin_pad, float *g, hipStream_t s);
 hipError_t mfma_probe(const float *A, const float *B, float *D, hipStream_t s);
 
 }  // namespace unet

@@ -61,7 +61,6 @@ class Net {
   size_t np;
   size_t w_off[10], b_off[10], head_w_off, head_b_off;
   bf16 *wf[10], *wt[10];
-  float *dw[10];  // gradient layout [9][cin_pad][cout]
   // activations
   bf16 *x32, *e1a, *e1, *p1, *e2a, *e2, *p2, *b1, *b2, *d2a, *d2, *d1a, *d1;
   float *out, *x12_scratch, *loss;
@@ -113,7 +112,6 @@ class Net {
       const LayerSpec &L = kLayers[l];
       wf[l] = (bf16 *)alloc((size_t)9 * L.cin_pad * L.cout * 2);
       wt[l] = l == 0 ? nullptr : (bf16 *)alloc((size_t)9 * L.cin_pad * L.cout * 2);
-      dw[l] = (float *)alloc((size_t)9 * L.cin_pad * L.cout * 4);
     }
     const size_t P = (size_t)n * h * w, P2 = P / 4, P4 = P / 16;
     auto A = [&](size_t pix, int c) { return (bf16 *)alloc(pix * c * 2); };
@@ -233,7 +231,8 @@ class Net {
     g.n = n;
     g.h = h;
     g.w = w;
-    g.dw = dw[l];
+    g.cin_log = kLayers[l].cin;
+    g.dw = grads + w_off[l];
     g.db = grads + b_off[l];
     g.part = wpart;
     g.part_bytes = wpart_bytes;
@@ -406,7 +405,9 @@ class Net {
     const size_t P = (size_t)n * h * w;
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
     forward(x12, n, h, w, out, s);
-    check(fill(grads, np, 0.f, s), "fill");  // the head's gradient is accumulated atomically
+    // the head's gradient and the loss are accumulated atomically; every
+    // convolution gradient is overwritten by its reduction
+    check(fill(grads + head_w_off, (size_t)kHeadIn * kHeadOut + kHeadOut, 0.f, s), "fill");
     check(fill(loss, 1, 0.f, s), "fill");
     check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P),
                    loss_kind == LGX_UNET_LOSS_L2, g_d1, grads + head_w_off, grads + head_b_off, loss, s),
@@ -440,17 +441,17 @@ class Net {
     wgrad(1, e1a, 32, 0, nullptr, 0, g_e1, n, h, w, s);
     dx_layer(1, n, h, w, s);
     wgrad(0, x32, 32, 0, nullptr, 0, g_e1a, n, h, w, s);
-    for (int l = 0; l < 10; l++)
-      check(unpack_wgrad(dw[l], kLayers[l].cout, kLayers[l].cin, kLayers[l].cin_pad, grads + w_off[l], s),
-            "unpack_wgrad");
   }
 
   void adam_step(float lr, hipStream_t s) {
     adam_t++;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float bc1 = 1.f - powf(b1, (float)adam_t), bc2 = 1.f - powf(b2, (float)adam_t);
-    check(adam(params, grads, adam_m, adam_v, np, lr, b1, b2, eps, bc1, bc2, s), "adam");
-    pack(s);
+    PackTable pk{};
+    pk.n = 10;
+    for (int l = 0; l < 10; l++)
+      pk.l[l] = PackLayer{w_off[l], kLayers[l].cout, kLayers[l].cin, kLayers[l].cin_pad, wf[l], wt[l]};
+    check(adam_pack(params, grads, adam_m, adam_v, np, lr, b1, b2, eps, bc1, bc2, pk, s), "adam_pack");
   }
 };
 

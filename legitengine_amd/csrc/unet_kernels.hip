@@ -767,30 +767,34 @@ template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(co
   }
 }
 
-// db[co] = sum over slots of part_b[slot][co], one workgroup per channel,
-// 256 strided partial sums then a fixed tree.
-__global__ __launch_bounds__(256) void bias_reduce_kernel(const float *part_b, int slots, int cout, float *db) {
-  __shared__ float red[256];
-  const int co = blockIdx.x;
-  float s = 0.f;
-  for (int k = threadIdx.x; k < slots; k += 256) s += part_b[(size_t)k * cout + co];
-  red[threadIdx.x] = s;
-  __syncthreads();
-#pragma unroll
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) db[co] = red[0];
-}
-
-// dw[tap][g * 32 + cl][co] = sum over slots of part[g][slot][tap][cl][co].
-// A workgroup holds 256 / S float4 elements x S slices; slice k sums slots
-// k, k + S, ... and the S slice sums are added in a fixed tree, so the
-// gradient does not depend on the order in which workgroups ran.
+// The fixed-order reductions of conv3x3_wgrad_kernel's per-workgroup partial
+// sums, in one launch.  Blocks [0, nwb): the weight gradient,
+//   dw[co][ci][tap] = sum over slots of part[g][slot][tap][ci % 32][co]
+// written in the master layout for ci < cin_log (g = ci / 32); a workgroup
+// holds 256 / S float4 elements x S slices, slice k sums slots k, k + S, ...
+// and the S slice sums are added in a fixed tree.  Block nwb + co: the bias
+// gradient db[co] = sum over slots of part_b[slot][co], 256 strided partial
+// sums then a fixed tree.  Neither result depends on the order in which
+// workgroups ran.
 template <int S>
 __global__ __launch_bounds__(256) void wgrad_reduce_kernel(const f32x4 *part, int slots, int per4, int groups,
-                                                          int cout, int cin, float *dw) {
+                                                          int cout, int cin_log, int nwb, const float *part_b,
+                                                          float *dw, float *db) {
+  if ((int)blockIdx.x >= nwb) {  // uniform per workgroup
+    __shared__ float redb[256];
+    const int co = (int)blockIdx.x - nwb;
+    float sb = 0.f;
+    for (int k = threadIdx.x; k < slots; k += 256) sb += part_b[(size_t)k * cout + co];
+    redb[threadIdx.x] = sb;
+    __syncthreads();
+#pragma unroll
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) redb[threadIdx.x] += redb[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = redb[0];
+    return;
+  }
   constexpr int E = 256 / S;
   __shared__ f32x4 red[S][E];
   const int el = threadIdx.x % E, sl = threadIdx.x / E;
@@ -811,7 +815,12 @@ __global__ __launch_bounds__(256) void wgrad_reduce_kernel(const f32x4 *part, in
   if (sl == 0 && e < total) {
     const int g = (int)(e / per4), f = (int)(e % per4) * 4;
     const int tap = f / (32 * cout), rem = f % (32 * cout), cl = rem / cout, co = rem % cout;
-    *(f32x4 *)&dw[((size_t)tap * cin + g * 32 + cl) * cout + co] = red[0][el];
+    const int ci = g * 32 + cl;
+    if (ci < cin_log) {
+      const f32x4 v = red[0][el];
+#pragma unroll
+      for (int i = 0; i < 4; i++) dw[((size_t)(co + i) * cin_log + ci) * 9 + tap] = v[i];
+    }
   }
 }
 
@@ -1014,16 +1023,32 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const flo
   }
 }
 
-// Adam on FP32 master parameters (Kingma & Ba 2015, bias-corrected).
-__global__ __launch_bounds__(256) void adam_kernel(float *p, const float *g, float *m, float *v, size_t n, float lr,
-                                                   float b1, float b2, float eps, float bc1, float bc2) {
+// Adam on the FP32 master parameters (Kingma & Ba 2015, bias-corrected);
+// every updated weight of the convolutions is also written to its BF16
+// forward and input-gradient layouts (pack_weights_kernel's placement; the
+// zero padding channels are never written).
+__global__ __launch_bounds__(256) void adam_pack_kernel(float *p, const float *g, float *m, float *v, size_t n, float lr,
+                                                        float b1, float b2, float eps, float bc1, float bc2,
+                                                        const PackTable pk) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    const float pi = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    p[i] = pi;
+#pragma unroll 1
+    for (int l = 0; l < pk.n; l++) {
+      const PackLayer &L = pk.l[l];
+      if (i < L.off || i >= L.off + (size_t)L.cout * L.cin_log * 9) continue;
+      const size_t r = i - L.off;
+      const int tap = (int)(r % 9), ci = (int)((r / 9) % L.cin_log), co = (int)(r / 9 / L.cin_log);
+      const bf16 b = f2bf(pi);
+      L.wf[(((size_t)tap * (L.cin_pad / 32) + ci / 32) * L.cout + co) * 32 + (ci & 31)] = b;
+      if (L.wt) L.wt[(((size_t)(8 - tap) * (L.cout / 32) + co / 32) * L.cin_pad + ci) * 32 + (co & 31)] = b;
+      break;
+    }
   }
 }
 
@@ -1045,18 +1070,6 @@ __global__ __launch_bounds__(256) void pack_weights_kernel(const float *w, int c
     const float v = ci < cin_log ? w[((size_t)co * cin_log + ci) * 9 + tap] : 0.f;
     wf[(((size_t)tap * (cin_pad / 32) + ci / 32) * cout + co) * 32 + (ci & 31)] = f2bf(v);
     if (wt) wt[(((size_t)(8 - tap) * (cout / 32) + co / 32) * cin_pad + ci) * 32 + (co & 31)] = f2bf(v);
-  }
-}
-
-// dW[tap][ci][co] (gradient layout) -> master layout [co][ci][tap] for ci < cin_log.
-__global__ __launch_bounds__(256) void unpack_wgrad_kernel(const float *dw, int cout, int cin_log, int cin_pad,
-                                                           float *g) {
-  const size_t total = (size_t)cout * cin_log * 9;
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
-    const int tap = (int)(e % 9);
-    const int ci = (int)((e / 9) % cin_log);
-    const int co = (int)(e / 9 / cin_log);
-    g[e] = dw[((size_t)tap * cin_pad + ci) * cout + co];
   }
 }
 
@@ -1250,7 +1263,8 @@ size_t conv3x3_wgrad_scratch_bytes() {
 
 hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
   if (a.cout != 32 && a.cout != 64 && a.cout != 128) return hipErrorInvalidValue;
-  if ((a.ca + a.cb) % 32 || a.ca % 32 || a.n < 1 || a.h < 1 || a.w < 1 || !a.part || !a.dw || !a.db)
+  if ((a.ca + a.cb) % 32 || a.ca % 32 || a.n < 1 || a.h < 1 || a.w < 1 || !a.part || !a.dw || !a.db ||
+      a.cin_log < 1 || a.cin_log > a.ca + a.cb)
     return hipErrorInvalidValue;
   const int groups = (a.ca + a.cb) / 32, bands = (a.w + 63) / 64, target = wgrad_target_blocks(a.cout);
   const long segs = (long)bands * a.n * groups;  // workgroups per row of rows
@@ -1278,17 +1292,16 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
   const int per4 = (int)(per / 4);
   const size_t total = (size_t)groups * per4;
   const f32x4 *part = (const f32x4 *)a.part;
-  const int cin = a.ca + a.cb;
   auto launch = [&](auto kern, int S) {
-    hipLaunchKernelGGL(kern, dim3((unsigned)((total + 256 / S - 1) / (256 / S))), dim3(256), 0, s, part, slots, per4,
-                       groups, a.cout, cin, a.dw);
+    const int nwb = (int)((total + 256 / S - 1) / (256 / S));
+    hipLaunchKernelGGL(kern, dim3((unsigned)(nwb + a.cout)), dim3(256), 0, s, part, slots, per4, groups, a.cout,
+                       a.cin_log, nwb, (const float *)a.part_b, a.dw, a.db);
   };
   // about 16 loads per thread
   if (slots <= 128) launch(wgrad_reduce_kernel<8>, 8);
   else if (slots <= 256) launch(wgrad_reduce_kernel<16>, 16);
   else if (slots <= 512) launch(wgrad_reduce_kernel<32>, 32);
   else launch(wgrad_reduce_kernel<64>, 64);
-  hipLaunchKernelGGL(bias_reduce_kernel, dim3(a.cout), dim3(256), 0, s, (const float *)a.part_b, slots, a.cout, a.db);
   return hipGetLastError();
 }
 
@@ -1333,11 +1346,13 @@ hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, con
                      dwh, dbh, loss);
   return hipGetLastError();
 }
-hipError_t adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
-                float bc1, float bc2, hipStream_t s) {
-  hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2);
+hipError_t adam_pack(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                     float bc1, float bc2, const PackTable &pk, hipStream_t s) {
+  hipLaunchKernelGGL(adam_pack_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1,
+                     bc2, pk);
   return hipGetLastError();
 }
+
 hipError_t fill(float *p, size_t n, float v, hipStream_t s) {
   hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
@@ -1345,11 +1360,6 @@ hipError_t fill(float *p, size_t n, float v, hipStream_t s) {
 hipError_t pack_weights(const float *w, int cout, int cin_log, int cin_pad, bf16 *wf, bf16 *wt, hipStream_t s) {
   hipLaunchKernelGGL(pack_weights_kernel, dim3(grid_for((size_t)cout * cin_pad * 9, 256)), dim3(256), 0, s, w, cout,
                      cin_log, cin_pad, wf, wt);
-  return hipGetLastError();
-}
-hipError_t unpack_wgrad(const float *dw, int cout, int cin_log, int cin_pad, float *g, hipStream_t s) {
-  hipLaunchKernelGGL(unpack_wgrad_kernel, dim3(grid_for((size_t)cout * cin_log * 9, 256)), dim3(256), 0, s, dw, cout,
-                     cin_log, cin_pad, g);
   return hipGetLastError();
 }
 hipError_t mfma_probe(const float *A, const float *B, float *D, hipStream_t s) {

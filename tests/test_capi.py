@@ -79,9 +79,9 @@ def test_gather_workspace_and_argument_checks():
     p = lgx.GatherParams()
     p.width, p.height, p.dirs, p.max_steps, p.grid, p.sampler = 1920, 1080, 32, 64, 4, 1
     n32 = L.lgx_gather_workspace_bytes(C.byref(p))
-    # 16 slices x (32 dirs x 2 + S steps x 2) floats + 11 level starts, with
+    # 16 slices x (32 dirs x 4 + S steps x 2) floats + 11 level starts, with
     # S <= 64 the steps the reference rule can take on a 1080p diagonal
-    rest = n32 - 16 * 32 * 2 * 4 - 16 * 11 * 4
+    rest = n32 - 16 * 32 * 4 * 4 - 16 * 11 * 4
     assert rest % (16 * 2 * 4) == 0 and 30 <= rest // (16 * 2 * 4) <= 64
     p.precision = lgx.PREC_F64
     assert L.lgx_gather_workspace_bytes(C.byref(p)) > n32

@@ -41,6 +41,8 @@ template <class T> struct GatherArgs {
   T e0[3], ex[3], ey[3];  // unnormalised ray through screen point s: e0 + s.x*ex + s.y*ey
   T W, H, near_step, log_ratio, ambient;
   int w, h, grid, grid_y, cells_x, cells_y, D, S_tab;  // pattern grid x grid_y
+  int d_pow2;      // D is a power of two: x / D == x * inv_d exactly
+  double inv_d;    // 1 / D
   T rW, rH;        // RN(1/W), RN(1/H) for the exact division by the extent
   const T *dirs;   // [grid * grid_y][D][4] = (cos, sin, 1 / cos, 1 / sin)
   const T *steps;  // [grid * grid_y][S_tab][2] = (offset, lod)
@@ -147,12 +149,52 @@ template <> struct RadBuf<LGX_RGBA32F> {
   }
 };
 
-// Monotone stand-in for atan2(y, x) on (-pi, pi]: values in [-2, 2].  Only
-// its order matters; near-ties decide steps whose contribution h(theta,
-// theta_max) vanishes with the gap.
-template <class T> LGX_D T pseudo_angle(T y, T x) {
-  T r = x * fast_rcp(fabs(x) + fabs(y));
-  return y >= T(0) ? T(1) - r : r - T(1);
+// Texel index iy * pitch + ix with the full-rate 24-bit multiply: rows and
+// pitches of a level stay below 2^24.  (A speculative index of an
+// off-screen step may wrap; the buffer resource's range check covers it.)
+LGX_D uint32_t texel_index(int iy, int pitch, int ix) {
+  return __umul24((unsigned)iy, (unsigned)pitch) + (uint32_t)ix;
+}
+
+// The running horizon maximum of a slice side and the test theta <
+// theta_max of :254, without a division per step.  The pseudo-angle
+// p(y, x) = 1 - x / s for y >= 0 and x / s - 1 for y < 0, s = |x| + |y|, is a
+// monotone stand-in for atan2 on (-pi, pi]; it is copysign(s - x, y) / s, so
+// with s > 0, p(y, x) < P is copysign(s - x, y) < s P: a subtraction, a
+// sign copy, a product and a compare per step, the division only when the
+// maximum P is replaced.  Only the order matters: near-ties decide steps
+// whose contribution h(theta, theta_max) vanishes with the gap.  A zero
+// vector has pseudo-angle 0 (atan2(0, 0) = 0) and is never below.
+template <class T> struct HorizonMax {
+  T P;
+  LGX_D void set(T y, T x) {
+    const T s = fabs(x) + fabs(y);
+    P = s > T(0) ? copysign(s - x, y) * fast_rcp(s) : T(0);
+  }
+  LGX_D bool below(T y, T x) const {
+    const T s = fabs(x) + fabs(y);
+    return copysign(s - x, y) < s * P;
+  }
+};
+
+// x / b for several x over one b, correctly rounded: r = RN(1/b) once (an
+// IEEE division), then per quotient y = RN(x r) and one fused correction
+// with the exact residual x - y b (Markstein), which rounds to the IEEE
+// quotient when r is the correctly rounded reciprocal (checked against the
+// IEEE division for every significand of b, tests/csrc/check_common_div.c);
+// x = 0 keeps its sign as the IEEE quotient does.  Three quotients cost one
+// division and three short corrections instead of three divisions.
+template <class T> LGX_D T div_by(T x, T b, T r) {
+  if (sizeof(T) == 8) return x / b;
+  const T y = x * r;
+  const T e = fma(-y, b, x);
+  const T q = fma(e, r, y);
+  return x == T(0) ? x : q;
+}
+template <class T> LGX_D Vec3<T> normalize3_cr(Vec3<T> v) {
+  const T len = sqrt(dot3(v, v));
+  const T r = T(1) / len;
+  return Vec3<T>{div_by(v.x, len, r), div_by(v.y, len, r), div_by(v.z, len, r)};
 }
 
 template <class T, class ARGS> LGX_D Vec3<T> unproject_far(ARGS &a, T sx, T sy) {
@@ -163,7 +205,8 @@ template <class T, class ARGS> LGX_D Vec3<T> unproject_far(ARGS &a, T sx, T sy) 
   T vy = m[1] * x + m[5] * y + m[9] * T(1) + m[13] * T(1);
   T vz = m[2] * x + m[6] * y + m[10] * T(1) + m[14] * T(1);
   T vw = m[3] * x + m[7] * y + m[11] * T(1) + m[15] * T(1);
-  return Vec3<T>{vx / vw, vy / vw, vz / vw};
+  const T rw = T(1) / vw;
+  return Vec3<T>{div_by(vx, vw, rw), div_by(vy, vw, rw), div_by(vz, vw, rw)};
 }
 
 template <int FMT> struct RadLoad {
@@ -248,10 +291,10 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
   const T rc = T(a.depth[(size_t)y * a.depth_pitch + x]);
   const float4 nt = Texel<NFMT>::load(a.normal, (size_t)y * a.normal_pitch + x);
   const Vec3<T> n = oct_decode<T>(T(nt.x), T(nt.y));  // :133
-  const Vec3<T> rayc = normalize3(unproject_far(a, scx, scy) - cam);
+  const Vec3<T> rayc = normalize3_cr(unproject_far(a, scx, scy) - cam);
   const Vec3<T> P = cam + rayc * rc;  // :130 from the radial distance
-  const Vec3<T> eye = normalize3(cam - P);  // :182
-  const Vec3<T> pdir = normalize3(P - cam);
+  const Vec3<T> eye = normalize3_cr(cam - P);  // :182
+  const Vec3<T> pdir = normalize3_cr(P - cam);
   const T ambient = a.ambient;
   // Ray through the sample, unnormalised, at the pixel centre (U) and per
   // unit of step offset along a direction (V is formed per direction).
@@ -271,14 +314,15 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
     // :180-183 tangent of the slice through the offset point (the quotient
     // by the extent is the IEEE one, as for the step positions)
     const T ox = div_extent(cx + dx * T(10e-1), a.W, a.rW), oy = div_extent(cy + dy * T(10e-1), a.H, a.rH);
-    const Vec3<T> offp = cam + normalize3(unproject_far(a, ox, oy) - cam) * rc;
-    const Vec3<T> tangent = normalize3(normalize3(offp - cam) - pdir);
+    const Vec3<T> offp = cam + normalize3_cr(unproject_far(a, ox, oy) - cam) * rc;
+    const Vec3<T> tangent = normalize3_cr(normalize3_cr(offp - cam) - pdir);
     // :193-198 normal-limited initial horizon
     const Vec3<T> c1 = cross3(tangent, eye);
     const Vec3<T> dn = cross3(Vec3<T>{-c1.x, -c1.y, -c1.z}, n);
     const T hy = dot3(dn, tangent), hx = dot3(dn, eye);
     T max_h = atan2_finite(hy, hx);
-    T pa_max = pseudo_angle(hy, hx);
+    HorizonMax<T> hm;
+    hm.set(hy, hx);
     const T iq2 = fast_rcp(hx * hx + hy * hy);
     T c2max = (hx * hx - hy * hy) * iq2, s2max = (T(2) * hx * hy) * iq2;
     const T tn = dot3(tangent, n);
@@ -356,8 +400,8 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
         // product can round up to the extent itself, hence the upper clamp
         const int ix = min((int)(f.sqx * flw), lw1);
         const int iy = min((int)(f.sqy * flh), lh1);
-        f.rti = (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix;
-        const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
+        f.rti = texel_index(iy, rpitch, ix);
+        const uint32_t ti = texel_index(iy, dpitch, ix);
         f.rd = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0));
         return f;
       };
@@ -387,10 +431,9 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
           const T r = T(f.rd);
           const T len2 = fma_(off, fma_(off, vv, uv2), uu);
           const T ri = r * fast_rsqrt(len2);
-          const T hyv = ri * fma_(off, tv, tu) - trc;
-          const T hxv = ri * fma_(off, ev, eu) - erc;
-          const T pa = pseudo_angle(hyv, hxv);
-          const bool hit = pa < pa_max;  // :254
+          const T hyv = fma_(ri, fma_(off, tv, tu), -trc);
+          const T hxv = fma_(ri, fma_(off, ev, eu), -erc);
+          const bool hit = hm.below(hyv, hxv);  // :254
           typename RadBuf<RFMT>::Raw rv;
           if (hit) rv = RadBuf<RFMT>::raw(rrs, f.rti);  // :256, on step j's level
           const T sqx = f.sqx, sqy = f.sqy;
@@ -422,7 +465,7 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
             L1 -= A(ambient * hc);
             L2 -= A(ambient * hc);
             max_h = th;  // :263
-            pa_max = pa;
+            hm.set(hyv, hxv);
             c2max = c2;
             s2max = s2;
           }
@@ -445,15 +488,14 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
           // round up to the extent itself, hence the upper clamp (in integers)
           const int ix = min((int)(sqx * flw), lw1);
           const int iy = min((int)(sqy * flh), lh1);
-          const uint32_t ti = (uint32_t)iy * (uint32_t)dpitch + (uint32_t)ix;
+          const uint32_t ti = texel_index(iy, dpitch, ix);
           const T r = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0)));
           // :241-252 sample delta projected on (eye, tangent)
           const T len2 = fma_(off, fma_(off, vv, uv2), uu);
           const T ri = r * fast_rsqrt(len2);
-          const T hyv = ri * fma_(off, tv, tu) - trc;
-          const T hxv = ri * fma_(off, ev, eu) - erc;
-          const T pa = pseudo_angle(hyv, hxv);
-          if (pa < pa_max) {  // :254
+          const T hyv = fma_(ri, fma_(off, tv, tu), -trc);
+          const T hxv = fma_(ri, fma_(off, ev, eu), -erc);
+          if (hm.below(hyv, hxv)) {  // :254
             if (COUNT) hit_total++;
             const T th = atan2_finite(hyv, hxv);
             const T iq = fast_rcp(hxv * hxv + hyv * hyv);
@@ -468,7 +510,7 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
             }
             const T hc =
                 (T(0.25) * en * (c2 - c2max) + T(0.25) * tn * (T(2) * max_h - T(2) * th - s2max + s2)) * side;
-            const float4 v = RadBuf<RFMT>::load(rrs, (uint32_t)iy * (uint32_t)rpitch + (uint32_t)ix);  // :256
+            const float4 v = RadBuf<RFMT>::load(rrs, texel_index(iy, rpitch, ix));  // :256
             L0 += A(T(v.x) * hc);  // :261
             L1 += A(T(v.y) * hc);
             L2 += A(T(v.z) * hc);
@@ -476,7 +518,7 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
             L1 -= A(ambient * hc);
             L2 -= A(ambient * hc);
             max_h = th;  // :263
-            pa_max = pa;
+            hm.set(hyv, hxv);
             c2max = c2;
             s2max = s2;
           }
@@ -504,8 +546,7 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
         const T ri = r * fast_rsqrt(len2);
         const T hyv = ri * fma_(off, tv, tu) - trc;
         const T hxv = ri * fma_(off, ev, eu) - erc;
-        const T pa = pseudo_angle(hyv, hxv);
-        if (pa < pa_max) {  // :254
+        if (hm.below(hyv, hxv)) {  // :254
             if (COUNT) hit_total++;
           const T th = atan2(hyv, hxv);
           const T qq = hxv * hxv + hyv * hyv;
@@ -537,16 +578,24 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
           L1 -= A(ambient * hc);
           L2 -= A(ambient * hc);
           max_h = th;  // :263
-          pa_max = pa;
+          hm.set(hyv, hxv);
           c2max = c2;
           s2max = s2;
         }
       }
     }
     if (COUNT) step_total += j;
-    sum0 += A(2) * L0 / A(a.D);  // :268
-    sum1 += A(2) * L1 / A(a.D);
-    sum2 += A(2) * L2 / A(a.D);
+    // :268; for a power-of-two D the quotient is the product with 1 / D
+    // (exact), otherwise the IEEE division
+    if (a.d_pow2) {
+      sum0 += A(2) * L0 * A(a.inv_d);
+      sum1 += A(2) * L1 * A(a.inv_d);
+      sum2 += A(2) * L2 * A(a.inv_d);
+    } else {
+      sum0 += A(2) * L0 / A(a.D);
+      sum1 += A(2) * L1 / A(a.D);
+      sum2 += A(2) * L2 / A(a.D);
+    }
   }
 
   int ox = x, oy = y;
@@ -883,6 +932,8 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
   a.cells_x = (p->width + a.grid - 1) / a.grid;
   a.cells_y = (p->height + a.grid_y - 1) / a.grid_y;
   a.D = p->dirs;
+  a.d_pow2 = (a.D & (a.D - 1)) == 0;
+  a.inv_d = 1.0 / a.D;
   a.S_tab = s.S_tab;
   a.dirs = (const T *)ws;
   a.steps = (const T *)ws + (size_t)s.g2 * s.D * 4;

@@ -141,6 +141,19 @@ def test_exact_extent_division(const_div_checker, extents):
     assert r.stdout.strip() == "ok", r.stdout
 
 
+def test_shared_denominator_division(tmp_path):
+    """The gather's normalisations and unprojection divide three numerators by
+    one denominator with one IEEE reciprocal and a Markstein correction per
+    quotient; every significand of the denominator (with fixed and random
+    numerators, signs and signed zeros) must give the IEEE quotient."""
+    exe = str(tmp_path / "check_common_div")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe,
+                    os.path.join(ROOT, "tests", "csrc", "check_common_div.c"), "-lm"], check=True)
+    r = subprocess.run([exe, "32"], capture_output=True, text=True, timeout=300)
+    assert r.stdout.startswith("ok "), r.stdout
+    assert int(r.stdout.split()[1]) > 3 * 10 ** 8
+
+
 @pytest.mark.skipif(os.environ.get("LGX_EXPECT_GPU") == "1", reason="checks behaviour without a device")
 def test_renderer_create_fails_cleanly_without_device():
     """Failure handling: no HIP device -> an error code, no crash, no handle."""

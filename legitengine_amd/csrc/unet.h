@@ -45,6 +45,14 @@ struct ConvArgs {
   // channels), summed as avgpool_kernel sums it; null = off.  Needs an LDS
   // tile whose waves hold an even number of rows (conv3x3_tile_pool).
   bf16 *pool_out;
+  // Input gradient of a layer whose input A is a nearest x2 upsample: for
+  // output channels [0, split) store, instead of the full-resolution
+  // gradient, its 2x2 sums at half resolution masked by up_mask > 0 (the
+  // gradient of the upsampled tensor, as upsample_bwd_kernel forms it from
+  // the stored values); [split, cout) are stored as usual.  Same tiles as
+  // pool_out; null = off.
+  bf16 *up_out;
+  const bf16 *up_mask;  // half resolution, split channels
 };
 
 struct WgradArgs {

@@ -200,8 +200,10 @@ class Net {
 
   // input gradient of layer l: dy (cout channels) -> [split | rest] with masks
   void conv_dx(int l, const bf16 *dy, int n, int h, int w, bf16 *o0, int split, bf16 *o1, const bf16 *m0,
-               const bf16 *m1, hipStream_t s) {
+               const bf16 *m1, hipStream_t s, bf16 *up_out = nullptr, const bf16 *up_mask = nullptr) {
     ConvArgs c{};
+    c.up_out = up_out;
+    c.up_mask = up_mask;
     c.a = dy;
     c.ca = kLayers[l].cout;
     c.n = n;
@@ -261,9 +263,23 @@ class Net {
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
     switch (l) {
       case 9: conv_dx(9, g_d1, n, h, w, g_d1a, 32, nullptr, d1a, nullptr, s); break;
-      case 8: conv_dx(8, g_d1a, n, h, w, g_upd2, 64, g_e1c, nullptr, nullptr, s); break;
+      case 8:  // [up(d2), e1]: the gradient of d2 through the upsample, in the epilogue when the tile allows
+        if (conv3x3_tile_pool(kLayers[8].cin_pad, dx_tile[8])) {
+          conv_dx(8, g_d1a, n, h, w, nullptr, 64, g_e1c, nullptr, nullptr, s, g_d2, d2);
+        } else {
+          conv_dx(8, g_d1a, n, h, w, g_upd2, 64, g_e1c, nullptr, nullptr, s);
+          check(upsample_bwd(g_upd2, d2, n, h2, w2, 64, g_d2, s), "upsample_bwd");
+        }
+        break;
       case 7: conv_dx(7, g_d2, n, h2, w2, g_d2a, 64, nullptr, d2a, nullptr, s); break;
-      case 6: conv_dx(6, g_d2a, n, h2, w2, g_upb2, 128, g_e2c, nullptr, nullptr, s); break;
+      case 6:  // [up(b2), e2]
+        if (conv3x3_tile_pool(kLayers[6].cin_pad, dx_tile[6])) {
+          conv_dx(6, g_d2a, n, h2, w2, nullptr, 128, g_e2c, nullptr, nullptr, s, g_b2, b2);
+        } else {
+          conv_dx(6, g_d2a, n, h2, w2, g_upb2, 128, g_e2c, nullptr, nullptr, s);
+          check(upsample_bwd(g_upb2, b2, n, h4, w4, 128, g_b2, s), "upsample_bwd");
+        }
+        break;
       case 5: conv_dx(5, g_b2, n, h4, w4, g_b1, 128, nullptr, b1, nullptr, s); break;
       case 4: conv_dx(4, g_b1, n, h4, w4, g_p2, 64, nullptr, nullptr, nullptr, s); break;
       case 3: conv_dx(3, g_e2, n, h2, w2, g_e2a, 64, nullptr, e2a, nullptr, s); break;
@@ -417,15 +433,13 @@ class Net {
     dx_layer(9, n, h, w, s);
     // d1a = relu(conv8([up(d2), e1]))
     wgrad(8, d2, 64, 1, e1, 32, g_d1a, n, h, w, s);
-    dx_layer(8, n, h, w, s);
-    check(upsample_bwd(g_upd2, d2, n, h2, w2, 64, g_d2, s), "upsample_bwd");
+    dx_layer(8, n, h, w, s);  // and the gradient of d2 through the upsample
     // d2 = relu(conv7(d2a))
     wgrad(7, d2a, 64, 0, nullptr, 0, g_d2, n, h2, w2, s);
     dx_layer(7, n, h, w, s);
     // d2a = relu(conv6([up(b2), e2]))
     wgrad(6, b2, 128, 1, e2, 64, g_d2a, n, h2, w2, s);
-    dx_layer(6, n, h, w, s);
-    check(upsample_bwd(g_upb2, b2, n, h4, w4, 128, g_b2, s), "upsample_bwd");
+    dx_layer(6, n, h, w, s);  // and the gradient of b2 through the upsample
     // b2 = relu(conv5(b1)), b1 = relu(conv4(p2))
     wgrad(5, b1, 128, 0, nullptr, 0, g_b2, n, h4, w4, s);
     dx_layer(5, n, h, w, s);

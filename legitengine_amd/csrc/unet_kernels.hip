@@ -80,7 +80,7 @@ LGX_D bf16x8 load8_clamped(const ConvInput &in, int n, int y, int x, int h, int 
 // (x0 + pb * 16 + col) x MB blocks of 16 channels (co0 + mb * 16 + kg * 4,
 // 4 per lane): bias (bias_r, this lane's channels), ReLU, the optional
 // mask (four channels in one read), the split into out0 / out1.
-template <int PB, int MB>
+template <int PB, int MB, bool UP = false>
 LGX_D void conv_store_row(const ConvArgs &a, const f32x4 (&acc)[PB][MB], const float (&bias_r)[MB][4], int n, int y,
                           int x0, int col, int kg, int co0) {
   typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -103,6 +103,7 @@ LGX_D void conv_store_row(const ConvArgs &a, const f32x4 (&acc)[PB][MB], const f
       // same for the whole 16-channel block: a uniform choice of pointers
       // (a per-lane choice would become a per-lane read of the arguments)
       const bool lo = co0 + mb * 16 < a.split;
+      if (UP && lo) continue;  // stored as 2x2 sums at half resolution
       bf16 *const dst = lo ? a.out0 : a.out1;
       const bf16 *const mask = lo ? a.mask0 : a.mask1;
       const int stride = lo ? a.split : a.cout - a.split, cch = lo ? co : co - a.split;
@@ -300,7 +301,9 @@ constexpr int conv_lds_blocks(int MB, int WR, int WP) {
   return 160 * 1024 / conv_lds_bytes(MB, WR, WP) < 3 ? 160 * 1024 / conv_lds_bytes(MB, WR, WP) : 3;
 }
 
-template <int MB, int WR, int WP, bool HEAD, int GIN = 0>
+// UP: the epilogue of an input gradient through an upsample (ConvArgs::up_out),
+// a separate instantiation so the other convolutions keep their registers.
+template <int MB, int WR, int WP, bool HEAD, int GIN = 0, bool UP = false>
 __global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_kernel(const ConvArgs a, int tiles_x,
                                                                                       int tiles_y, int splits) {
   constexpr int TH = 4 * WR, TW = 16 * WP, IW = TW + 2, IH = TH + 2;
@@ -582,11 +585,57 @@ __global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_
         }
       }
     }
+    if constexpr (UP) {
+      // channels [0, split): 2x2 sums of the stored (BF16) values in
+      // upsample_bwd_kernel's order (y, x), (y, x+1), (y+1, x), (y+1, x+1),
+      // masked at half resolution; rows r, r + 1 are this lane's, columns
+      // x, x + 1 are lanes col, col + 1
+      const int hh = a.h >> 1, ww = a.w >> 1;
+#pragma unroll
+      for (int r = 0; r + 1 < WR; r += 2) {
+        const int y = cur.y0 + wave * WR + r;  // even
+#pragma unroll
+        for (int pb = 0; pb < WP; pb++) {
+          const int x = cur.x0 + pb * 16 + col;
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) {
+            if (!(cur.co0 + mb * 16 < a.split)) continue;  // uniform
+            const int co = cur.co0 + mb * 16 + kg * 4;
+            float t[4], b[4], tn[4], bn[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              float v0 = acc[r][pb][mb][i], v1 = acc[r + 1][pb][mb][i];
+              if (a.bias) {
+                v0 += bias_r[mb][i];
+                v1 += bias_r[mb][i];
+              }
+              if (a.relu) {
+                v0 = v0 > 0.f ? v0 : 0.f;
+                v1 = v1 > 0.f ? v1 : 0.f;
+              }
+              t[i] = bf2f(f2bf(v0));
+              b[i] = bf2f(f2bf(v1));
+              tn[i] = lane_xor1(t[i]);
+              bn[i] = lane_xor1(b[i]);
+            }
+            if (!(col & 1) && (y >> 1) < hh && (x >> 1) < ww) {
+              typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+              const size_t q = (((size_t)cur.n * hh + (y >> 1)) * ww + (x >> 1)) * a.split + co;
+              const bf16x4 m = *(const bf16x4 *)(a.up_mask + q);
+              bf16x4 o;
+#pragma unroll
+              for (int i = 0; i < 4; i++) o[i] = bf2f(m[i]) > 0.f ? f2bf(((t[i] + tn[i]) + b[i]) + bn[i]) : f2bf(0.f);
+              *(bf16x4 *)(a.up_out + q) = o;
+            }
+          }
+        }
+      }
+    }
 #pragma unroll
     for (int r = 0; r < WR; r++) {
       const int y = cur.y0 + wave * WR + r;
       if (y >= a.h) break;
-      conv_store_row<WP, MB>(a, acc[r], bias_r, cur.n, y, cur.x0, col, kg, cur.co0);
+      conv_store_row<WP, MB, UP>(a, acc[r], bias_r, cur.n, y, cur.x0, col, kg, cur.co0);
     }
     }
     t += gridDim.x;
@@ -612,9 +661,8 @@ __global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_
 // wgrad_reduce_kernel adds the slots in a fixed order.  The workgroups of
 // input group 0 also sum the staged output-gradient rows per channel (the
 // bias gradient) into part_b[slot][co], reduced by bias_reduce_kernel.
-template <int MB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const WgradArgs a) {
+template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const WgradArgs a) {
   constexpr int XS = 64 + 16;  // staged x range [x0 - 8, x0 + 72): keeps dx = 0 reads 16-byte aligned
-  constexpr int RB = 2;        // output rows per staging round
   constexpr int COUT = MB * 16;
   __shared__ __align__(16) bf16 xs[RB + 2][32][XS];
   __shared__ __align__(16) bf16 ds[RB][COUT][64];
@@ -1191,6 +1239,13 @@ template <int MBW, int WR, int WP> static void launch_conv_lds(const ConvArgs &a
       return;
     }
   }
+  if constexpr (WR % 2 == 0) {
+    if (a.up_out) {
+      hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, false, 0, true>), dim3((unsigned)grid), dim3(256), 0, s, a,
+                         tx, ty, splits);
+      return;
+    }
+  }
   hipLaunchKernelGGL((conv3x3_lds_kernel<MBW, WR, WP, false>), dim3((unsigned)grid), dim3(256), 0, s, a, tx, ty,
                      splits);
 }
@@ -1234,6 +1289,9 @@ hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
     return hipErrorInvalidValue;
   if (a.pool_out && (a.head_out || a.mask0 || a.mask1 || !conv3x3_tile_pool(a.cout, a.tile)))
     return hipErrorInvalidValue;
+  if (a.up_out && (a.head_out || a.pool_out || a.mask0 || !a.up_mask || a.split < 16 || a.h % 2 || a.w % 2 ||
+                   !conv3x3_tile_pool(a.cout, a.tile)))
+    return hipErrorInvalidValue;
   if (a.split < 0 || a.split > a.cout || a.split % 16) return hipErrorInvalidValue;  // whole 16-channel blocks
   const int mb = a.cout / 16;
   int n = 0;
@@ -1254,6 +1312,12 @@ hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
 // Weight-gradient workgroups per launch by output-channel count: each stores
 // 9 * 32 * cout FP32 partial sums, about 38 MB per launch at these counts.
 static int wgrad_target_blocks(int cout) { return cout <= 32 ? 1024 : cout <= 64 ? 512 : 256; }
+// Output rows staged per round of conv3x3_wgrad_kernel: a round's loads are
+// in flight during the previous round's matrix work, so taller rounds expose
+// that latency fewer times; the 128-channel kernel (the most registers)
+// keeps 2.
+static constexpr int kWgradRB2 = 4, kWgradRB4 = 4, kWgradRB8 = 2;
+static int wgrad_rb(int cout) { return cout <= 32 ? kWgradRB2 : cout <= 64 ? kWgradRB4 : kWgradRB8; }
 
 size_t conv3x3_wgrad_scratch_bytes() {
   size_t m = 0;
@@ -1268,12 +1332,14 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
     return hipErrorInvalidValue;
   const int groups = (a.ca + a.cb) / 32, bands = (a.w + 63) / 64, target = wgrad_target_blocks(a.cout);
   const long segs = (long)bands * a.n * groups;  // workgroups per row of rows
-  // rows per workgroup: even, about `target` workgroups, at most 2 * target
-  int rows = (int)std::min<long>(a.h, std::max<long>(2, ((long)a.h * segs + target - 1) / target));
-  rows = (rows + 1) & ~1;
+  // rows per workgroup: a multiple of the round height, about `target`
+  // workgroups, at most 2 * target
+  const int rb = wgrad_rb(a.cout);
+  int rows = (int)std::min<long>(a.h, std::max<long>(rb, ((long)a.h * segs + target - 1) / target));
+  rows = (rows + rb - 1) / rb * rb;
   long blocks = segs * ((a.h + rows - 1) / rows);
   while (blocks > 2L * target && rows < a.h) {
-    rows += 2;
+    rows += rb;
     blocks = segs * ((a.h + rows - 1) / rows);
   }
   const size_t per = (size_t)9 * 32 * a.cout;
@@ -1283,9 +1349,9 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
   a.part_b = a.part + (size_t)blocks * per;
   dim3 grid(bands, (a.h + rows - 1) / rows, a.n * groups), block(256);
   switch (a.cout / 16) {
-    case 2: hipLaunchKernelGGL(conv3x3_wgrad_kernel<2>, grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL(conv3x3_wgrad_kernel<4>, grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL(conv3x3_wgrad_kernel<8>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((conv3x3_wgrad_kernel<2, kWgradRB2>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((conv3x3_wgrad_kernel<4, kWgradRB4>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((conv3x3_wgrad_kernel<8, kWgradRB8>), grid, block, 0, s, a); break;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -105,7 +105,7 @@ class Net {
     off += kHeadIn * kHeadOut;
     head_b_off = off;
     params = (float *)alloc(np * 4);
-    grads = (float *)alloc(np * 4);
+    grads = (float *)alloc((np + 1) * 4);  // and the loss after the head's gradient
     adam_m = (float *)alloc(np * 4);
     adam_v = (float *)alloc(np * 4);
     for (int l = 0; l < 10; l++) {
@@ -130,7 +130,7 @@ class Net {
     d1 = A(P, 32);
     out = (float *)alloc(P * 16);
     x12_scratch = (float *)alloc((size_t)h * w * 48);
-    loss = (float *)alloc(4);
+    loss = grads + np;
     wpart_bytes = conv3x3_wgrad_scratch_bytes();
     wpart = (float *)alloc(wpart_bytes);
     g_d1 = A(P, 32);
@@ -423,8 +423,7 @@ class Net {
     forward(x12, n, h, w, out, s);
     // the head's gradient and the loss are accumulated atomically; every
     // convolution gradient is overwritten by its reduction
-    check(fill(grads + head_w_off, (size_t)kHeadIn * kHeadOut + kHeadOut, 0.f, s), "fill");
-    check(fill(loss, 1, 0.f, s), "fill");
+    check(fill(grads + head_w_off, (size_t)kHeadIn * kHeadOut + kHeadOut + 1, 0.f, s), "fill");  // and the loss
     check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P),
                    loss_kind == LGX_UNET_LOSS_L2, g_d1, grads + head_w_off, grads + head_b_off, loss, s),
           "head_bwd");

@@ -664,12 +664,32 @@ __global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_
 template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const WgradArgs a) {
   constexpr int XS = 64 + 16;  // staged x range [x0 - 8, x0 + 72): keeps dx = 0 reads 16-byte aligned
   constexpr int COUT = MB * 16;
-  __shared__ __align__(16) bf16 xs[RB + 2][32][XS];
-  __shared__ __align__(16) bf16 ds[RB][COUT][64];
+  // LDS layouts.  A 128-bit operand read is served 16 lanes at a time, lanes
+  // col = 0..7 and col = 8..15 of one kg together: with channel rows 44 banks
+  // apart, lanes 0..7 start on 8 banks 4 apart and lanes 8..15 on the other
+  // 8, so the 16 lanes cover the 64 banks once.  A staging store writes one
+  // 32-bit pixel pair per lane, the lanes differing in their 8-channel group
+  // and pixel pair: channel rows 44 banks apart put 8-channel groups 0 or 32
+  // banks apart, and a pad of 512 / COUT banks per 16-channel group (16 for
+  // the 32 input channels) separates the rest.  A staged input row holds 32
+  // channel rows of XS pixels and is 40 banks longer than a multiple of 64,
+  // so a store whose pixel pairs run past the end of a row stays conflict
+  // free.
+  constexpr int CST = 88;                                   // channel-row stride (bf16), XS and 64 pixels used
+  constexpr int XROW = 2896;                                // >= 32 * CST + 2 * 32, 1448 banks = 40 mod 64
+  constexpr int DPAD = 1024 / COUT, DROW = COUT * CST + (COUT / 16) * DPAD;
+  static_assert(XROW >= 32 * CST + 2 * 32 && (XROW / 2) % 64 == 40, "input row layout");
+  __shared__ __align__(16) bf16 xs[(RB + 2) * XROW];
+  __shared__ __align__(16) bf16 ds[RB * DROW];
+  auto xi = [](int rr, int c, int px) { return rr * XROW + c * CST + (c >> 4) * 32 + px; };
+  auto di = [](int rr, int c, int px) { return rr * DROW + c * CST + (c >> 4) * DPAD + px; };
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kg = lane >> 4;
   const int groups = (a.ca + a.cb) >> 5;
-  const int n = blockIdx.z / groups, g = blockIdx.z % groups;
+  // a workgroup computes COUT of the a.cout output channels: channel slice hf
+  const int halves = a.cout / COUT;
+  const int hf = blockIdx.z % halves, zc = blockIdx.z / halves;
+  const int n = zc / groups, g = zc % groups;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * a.rows;
   const ConvInput in{a.a, a.ca, a.a_up, a.b, a.cb};
   constexpr int MJ = MB / 2;  // output blocks per wave
@@ -713,9 +733,9 @@ template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_k
       const int e = min((int)threadIdx.x + i * 256, RB * 32 * (COUT / 8) - 1);
       const int q = e % (COUT / 8), pp = (e / (COUT / 8)) % 32, rr = e / (COUT / 8) / 32;
       const int y = min(yr + rr, a.h - 1), x = x0 + pp * 2;
-      const bf16 *row = a.dy + ((size_t)n * a.h + y) * a.w * COUT + q * 8;
-      pd0[i] = *(const bf16x8 *)(row + (size_t)min(x, a.w - 1) * COUT);
-      pd1[i] = *(const bf16x8 *)(row + (size_t)min(x + 1, a.w - 1) * COUT);
+      const bf16 *row = a.dy + ((size_t)n * a.h + y) * a.w * a.cout + hf * COUT + q * 8;
+      pd0[i] = *(const bf16x8 *)(row + (size_t)min(x, a.w - 1) * a.cout);
+      pd1[i] = *(const bf16x8 *)(row + (size_t)min(x + 1, a.w - 1) * a.cout);
       okd |= (uint32_t)(rr < nr && x < a.w) << (2 * i);
       okd |= (uint32_t)(rr < nr && x + 1 < a.w) << (2 * i + 1);
     }
@@ -734,7 +754,7 @@ template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_k
         const bf16x8 z = {};
         const bf16x8 v0 = (okx >> (2 * i)) & 1u ? px0[i] : z, v1 = (okx >> (2 * i + 1)) & 1u ? px1[i] : z;
 #pragma unroll
-        for (int k = 0; k < 8; k++) *(uint32_t *)&xs[rr][q * 8 + k][pp * 2] = pair(v0[k], v1[k]);
+        for (int k = 0; k < 8; k++) *(uint32_t *)&xs[xi(rr, q * 8 + k, pp * 2)] = pair(v0[k], v1[k]);
       }
     }
     // output-gradient rows yr .. yr + nr - 1, transposed, pixel pairs
@@ -746,7 +766,7 @@ template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_k
         const bf16x8 z = {};
         const bf16x8 v0 = (okd >> (2 * i)) & 1u ? pd0[i] : z, v1 = (okd >> (2 * i + 1)) & 1u ? pd1[i] : z;
 #pragma unroll
-        for (int k = 0; k < 8; k++) *(uint32_t *)&ds[rr][q * 8 + k][pp * 2] = pair(v0[k], v1[k]);
+        for (int k = 0; k < 8; k++) *(uint32_t *)&ds[di(rr, q * 8 + k, pp * 2)] = pair(v0[k], v1[k]);
       }
     }
     __syncthreads();
@@ -758,7 +778,7 @@ template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_k
       for (int rr = 0; rr < nr; rr++)
 #pragma unroll
         for (int k = 0; k < BW; k += 8) {
-          const bf16x8 v = *(const bf16x8 *)&ds[rr][bc][bq * BW + k];
+          const bf16x8 v = *(const bf16x8 *)&ds[di(rr, bc, bq * BW + k)];
 #pragma unroll
           for (int i = 0; i < 8; i++) bsum += bf2f(v[i]);
         }
@@ -769,10 +789,10 @@ template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_k
         const int p0 = kc * 32 + kg * 8;
         bf16x8 av[MJ];  // A: co x pixels
 #pragma unroll
-        for (int j = 0; j < MJ; j++) av[j] = *(const bf16x8 *)&ds[rr][(mb0 + 2 * j) * 16 + col][p0];
+        for (int j = 0; j < MJ; j++) av[j] = *(const bf16x8 *)&ds[di(rr, (mb0 + 2 * j) * 16 + col, p0)];
 #pragma unroll
         for (int dy = 0; dy < 3; dy++) {
-          const bf16 *base = &xs[rr + dy][cb * 16 + col][p0 + 8];  // B: pixels x ci, 16-byte aligned
+          const bf16 *base = &xs[xi(rr + dy, cb * 16 + col, p0 + 8)];  // B: pixels x ci, 16-byte aligned
           const bf16x8 lo = *(const bf16x8 *)(base - 8), mid = *(const bf16x8 *)base,
                        hi = *(const bf16x8 *)(base + 8);
           bf16x8 bm, bp;  // dx = -1: elements base[-1 .. 6]; dx = +1: base[1 .. 8]
@@ -795,22 +815,22 @@ template <int MB, int RB> __global__ __launch_bounds__(256) void conv3x3_wgrad_k
   }
   const size_t slots = (size_t)a.n * gridDim.y * gridDim.x;
   const size_t slot = ((size_t)n * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  float *dst = a.part + ((size_t)g * slots + slot) * (9 * 32 * COUT);
+  float *dst = a.part + ((size_t)g * slots + slot) * (9 * 32 * a.cout) + hf * COUT;
 #pragma unroll
   for (int tap = 0; tap < 9; tap++)
 #pragma unroll
     for (int j = 0; j < MJ; j++)
-      *(f32x4 *)&dst[(tap * 32 + cb * 16 + col) * COUT + (mb0 + 2 * j) * 16 + kg * 4] = acc[tap][j];
+      *(f32x4 *)&dst[(tap * 32 + cb * 16 + col) * a.cout + (mb0 + 2 * j) * 16 + kg * 4] = acc[tap][j];
   if (g == 0) {
     // the BP slices of each channel, added in a fixed order (xs is free now)
-    float *red = (float *)&xs[0][0][0];
+    float *red = (float *)&xs[0];
     red[bq * COUT + bc] = bsum;
     __syncthreads();
     if (threadIdx.x < COUT) {
       float b = 0.f;
 #pragma unroll
       for (int k = 0; k < BP; k++) b += red[k * COUT + threadIdx.x];
-      a.part_b[slot * COUT + threadIdx.x] = b;
+      a.part_b[slot * a.cout + hf * COUT + threadIdx.x] = b;
     }
   }
 }
@@ -1311,13 +1331,18 @@ hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
 
 // Weight-gradient workgroups per launch by output-channel count: each stores
 // 9 * 32 * cout FP32 partial sums, about 38 MB per launch at these counts.
-static int wgrad_target_blocks(int cout) { return cout <= 32 ? 1024 : cout <= 64 ? 512 : 256; }
+// Workgroups of conv3x3_wgrad_kernel: about the number that are resident at
+// once (2 per CU: the accumulators live in AGPRs, which share the register
+// file with the VGPRs, and LDS), so that the launch is one round of
+// workgroups rather than a round and a fraction with most CUs idle in the
+// fraction.
+static int wgrad_target_blocks(int cout) { return (cout == 128 ? 1 : 2) * cu_count(); }  // x2 slices for 128
 // Output rows staged per round of conv3x3_wgrad_kernel: a round's loads are
 // in flight during the previous round's matrix work, so taller rounds expose
-// that latency fewer times; the 128-channel kernel (the most registers)
-// keeps 2.
-static constexpr int kWgradRB2 = 4, kWgradRB4 = 4, kWgradRB8 = 2;
-static int wgrad_rb(int cout) { return cout <= 32 ? kWgradRB2 : cout <= 64 ? kWgradRB4 : kWgradRB8; }
+// that latency fewer times, within two workgroups' LDS per CU (64 channels:
+// 3 rows).
+static constexpr int kWgradRB2 = 4, kWgradRB4 = 3;
+static int wgrad_rb(int cout) { return cout <= 32 ? kWgradRB2 : kWgradRB4; }
 
 size_t conv3x3_wgrad_scratch_bytes() {
   size_t m = 0;
@@ -1347,11 +1372,14 @@ hipError_t conv3x3_wgrad(WgradArgs a, hipStream_t s) {
   if (((size_t)blocks * per + (size_t)slots * a.cout) * 4 > a.part_bytes) return hipErrorInvalidValue;
   a.rows = rows;
   a.part_b = a.part + (size_t)blocks * per;
-  dim3 grid(bands, (a.h + rows - 1) / rows, a.n * groups), block(256);
+  // 128 output channels: the 64-channel kernel on two channel slices (two
+  // workgroups per pixel block; the 128-channel accumulators would leave one
+  // wave per SIMD)
+  const int halves = a.cout == 128 ? 2 : 1;
+  dim3 grid(bands, (a.h + rows - 1) / rows, a.n * groups * halves), block(256);
   switch (a.cout / 16) {
     case 2: hipLaunchKernelGGL((conv3x3_wgrad_kernel<2, kWgradRB2>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((conv3x3_wgrad_kernel<4, kWgradRB4>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((conv3x3_wgrad_kernel<8, kWgradRB8>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((conv3x3_wgrad_kernel<4, kWgradRB4>), grid, block, 0, s, a); break;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

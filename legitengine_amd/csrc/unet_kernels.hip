@@ -1022,9 +1022,10 @@ __global__ __launch_bounds__(256) void head_fwd_img_kernel(const bf16 *d1, const
 // (one 16-byte load and store), so a thread keeps 3 x 8 head-weight partial
 // sums; they are reduced over the lanes of a wave with shuffles, over the
 // waves in LDS, and added to dW_h, db_h once per workgroup.
-__global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const float4 *out, const float4 *target,
-                                                       const float *wh, size_t npix, float inv_count, int l2,
-                                                       bf16 *dd1, float *dwh, float *dbh, float *loss) {
+__global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *__restrict__ d1, const float4 *__restrict__ out,
+                                                       const float4 *__restrict__ target, const float *__restrict__ wh,
+                                                       size_t npix, float inv_count, int l2, bf16 *__restrict__ dd1,
+                                                       float *dwh, float *dbh, float *loss) {
   __shared__ float red[4][100];
   const int q = threadIdx.x & 3, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float w8[3][8], aw[3][8], ab[3] = {0.f, 0.f, 0.f}, al = 0.f;
@@ -1035,30 +1036,46 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *d1, const flo
       w8[c][i] = wh[c * 32 + q * 8 + i];
       aw[c][i] = 0.f;
     }
-  for (size_t p = ((size_t)blockIdx.x * 256 + threadIdx.x) >> 2; p < npix; p += (size_t)gridDim.x * 64) {
-    const float4 o = out[p], t = target[p];
-    const float df[3] = {o.x - t.x, o.y - t.y, o.z - t.z};
-    float g[3];
+  // PB pixels per thread and pass, their loads issued together (pixels of a
+  // pass are a grid stride apart; the sums run over them in that order)
+  constexpr int PB = 4;
+  const size_t stride = (size_t)gridDim.x * 64;
+  for (size_t p0 = ((size_t)blockIdx.x * 256 + threadIdx.x) >> 2; p0 < npix; p0 += PB * stride) {
+    float4 o[PB], t[PB];
+    bf16x8 v[PB];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      g[c] = (l2 ? 2.f * df[c] : (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f))) * inv_count;
-      al += l2 ? df[c] * df[c] : fabsf(df[c]);
-      ab[c] += g[c];
+    for (int k = 0; k < PB; k++) {
+      const size_t p = min(p0 + k * stride, npix - 1);  // in bounds; pixels past the end are not used
+      o[k] = out[p];
+      t[k] = target[p];
+      v[k] = *(const bf16x8 *)(d1 + p * 32 + q * 8);
     }
-    const bf16x8 v = *(const bf16x8 *)(d1 + p * 32 + q * 8);
-    bf16x8 o8;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const float f = bf2f(v[i]);
-      float sacc = 0.f;
+    for (int k = 0; k < PB; k++) {
+      const size_t p = p0 + k * stride;
+      if (p >= npix) break;
+      const float df[3] = {o[k].x - t[k].x, o[k].y - t[k].y, o[k].z - t[k].z};
+      float g[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        sacc += w8[c][i] * g[c];
-        aw[c][i] += g[c] * f;
+        g[c] = (l2 ? 2.f * df[c] : (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f))) * inv_count;
+        al += l2 ? df[c] * df[c] : fabsf(df[c]);
+        ab[c] += g[c];
       }
-      o8[i] = f > 0.f ? f2bf(sacc) : f2bf(0.f);
+      bf16x8 o8;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float f = bf2f(v[k][i]);
+        float sacc = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          sacc += w8[c][i] * g[c];
+          aw[c][i] += g[c] * f;
+        }
+        o8[i] = f > 0.f ? f2bf(sacc) : f2bf(0.f);
+      }
+      *(bf16x8 *)(dd1 + p * 32 + q * 8) = o8;
     }
-    *(bf16x8 *)(dd1 + p * 32 + q * 8) = o8;
   }
   // lanes with the same q hold the same channels: reduce over lane bits 2..5
 #pragma unroll

@@ -129,21 +129,15 @@ def main():
     gather_bytes = total_steps * BD + total_hits * BR + W * H * (BC + BO)
     gather_bytes_per_step_model = total_steps * (BD + BR) + W * H * (BC + BO)
 
-    for _ in range(args.warmup):
-        renderer.render(g, view, proj)
-    torch.cuda.synchronize()
-    renderer.pass_times(mode=2)  # reset the per-pass accumulation
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        renderer.render(g, view, proj)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    passes = renderer.pass_times(mode=1)
+    # per-pass times from eager frames (a hipEvent pair around every pass);
+    # the frame time from the same frame captured once into a hipGraph and
+    # replayed, the renderer's submission path (the per-frame kernel launches
+    # and events of the eager frame are host overhead, reported beside it)
+    ms_per_step, passes, eager_ms = time_renderer(renderer, g, view, proj, args.steps, args.warmup)
     gather_ms = sum(p["ms"] for p in passes if p["name"] == "IndirectLightPass")
     achieved = gather_bytes / (gather_ms * 1e-3) / 1e9
 
-    log(f"plain path: {ms_per_step:.3f} ms/frame, gather {gather_ms:.3f} ms")
+    log(f"plain path: {ms_per_step:.3f} ms/frame (graph replay; eager {eager_ms:.3f}), gather {gather_ms:.3f} ms")
     traffic = read_traffic(args.traffic_file)
     result = {
         "metric": "GI ms/frame @1080p: plain vs denoised at equal rel-L2 vs FP64 oracle",
@@ -174,6 +168,8 @@ def main():
                                            if traffic else None),
                      **read_counters(args.counters_file)},
         "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(passes)},
+        "frame_submission": "frame captured once into a hipGraph (lgx_renderer_capture) and replayed per frame",
+        "eager_ms_per_frame": round(eager_ms, 4),
     }
 
     if not args.no_denoised:
@@ -296,7 +292,26 @@ def read_counters(path):
     return res
 
 
+def time_replay(r, g, view, proj, steps, warmup):
+    """ms per frame of the renderer's frame captured once into a hipGraph
+    (lgx_renderer_capture: the same passes, no per-pass events) and replayed
+    `steps` times on a side stream, bracketed by device synchronisations."""
+    import torch
+    st = torch.cuda.Stream()
+    r.capture(g, view, proj, st)
+    for _ in range(warmup):
+        r.replay(st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.replay(st)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
 def time_renderer(r, g, view, proj, steps, warmup):
+    """(ms per frame of the replayed frame graph, per-pass times of eager
+    frames with a hipEvent pair around every pass, ms per eager frame)."""
     import torch
     for _ in range(warmup):
         r.render(g, view, proj)
@@ -306,7 +321,9 @@ def time_renderer(r, g, view, proj, steps, warmup):
     for _ in range(steps):
         r.render(g, view, proj)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) * 1e3 / steps, r.pass_times(mode=1)
+    eager = (time.perf_counter() - t0) * 1e3 / steps
+    passes = r.pass_times(mode=1)
+    return time_replay(r, g, view, proj, steps, warmup), passes, eager
 
 
 def rel_l2(a, b):
@@ -347,12 +364,13 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     rden = dev.Renderer(W, H, DEN_DIRS, dn.NOISY_STEPS, LEVELS, sampler, dn.NOISY_GRID, denoiser=lgx.DENOISER_UNET,
                         kind="mixed", profile=True, footprint_dirs=dn.NOISY_FOOTPRINT, tile=den_tile)
     rden.attach_unet(net)
-    den_ms, den_passes = time_renderer(rden, g, view, proj, args.steps, args.warmup)
-    log(f"denoised renderer timed: {den_ms:.3f} ms/frame")
+    den_ms, den_passes, den_eager = time_renderer(rden, g, view, proj, args.steps, args.warmup)
+    log(f"denoised renderer timed: {den_ms:.3f} ms/frame (eager {den_eager:.3f})")
     unet_ms = sum(p["ms"] for p in den_passes if p["name"] == "DenoiserPass")
     flops = dn.unet_flops_per_pixel() * W * H
     timing = {
         "ms_per_frame": round(den_ms, 4),
+        "eager_ms_per_frame": round(den_eager, 4),
         "unet_ms": round(unet_ms, 4),
         "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(den_passes)},
         "unet_roofline": {"bound": "mfma", "achieved": round(flops / (unet_ms * 1e-3) / 1e12, 2),
@@ -370,7 +388,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         for kind, den in (("plain", lgx.DENOISER_BILATERAL), ("raw", lgx.DENOISER_NONE)):
             r = dev.Renderer(W, H, d, STEPS if d <= DIRS else 0, LEVELS, sampler, GRID, denoiser=den,
                              denoise_passes=BILAT_PASSES, denoise_radius=BILAT_RADIUS, kind="mixed", profile=True)
-            ms, _ = time_renderer(r, g, view, proj, max(3, args.steps // 4), 1)
+            ms, _, _ = time_renderer(r, g, view, proj, max(3, args.steps // 4), 1)
             plain[f"{kind}{d}"] = {"renderer": r, "ms": ms, "err": []}
             log(f"{kind}{d}: {ms:.3f} ms/frame")
     den_err, noisy_err = [], []

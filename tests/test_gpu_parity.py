@@ -259,6 +259,20 @@ def test_bilateral(dev, kind, w, h):
     assert z.numpy().tobytes() == inp.tobytes()
 
 
+@pytest.mark.parametrize("radius", [1, 2, 5, 8])
+def test_bilateral_radii(dev, radius):
+    """Both tile shapes of the filter (32x32 up to r = 4, 16x16 above) and
+    their strip widths, at an extent that is a multiple of neither."""
+    w, h = 200, 136
+    sc, view, proj, gb = host_scene(4, w, h, "mixed", 8, 2)
+    g = dev.GBufferDev.from_host(gb)
+    inp = np.random.default_rng(radius).random((h, w, 4), dtype=np.float32)
+    out = dev.bilateral(dev.DImage.from_numpy(inp, lgx.RGBA32F), g, radius=radius)
+    torch.cuda.synchronize()
+    ref = orc.bilateral(inp, gb["depth"], orc.widen(gb["normal"]), radius)
+    assert orc.rel_l2(out.numpy(), ref) <= BILAT_TOL
+
+
 def test_regression_denoise(dev):
     sc, view, proj, gb = host_scene(4, 128, 96, "f32", 8, 2)
     g = dev.GBufferDev.from_host(gb)

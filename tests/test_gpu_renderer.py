@@ -91,6 +91,30 @@ def test_capture_replay_matches_eager(dev):
     assert r2.output().tobytes() == eager.tobytes()
 
 
+def test_capture_replay_unet_frame(dev):
+    """The denoised frame (4-direction gather on the 8x8 pattern, U-Net
+    DenoiserPass, profiling on) captured into a graph: every replay, with the
+    program's own work in between, equals the eager frame bit for bit."""
+    w, h = 320, 192
+    view, proj, gb, g = frame_inputs(dev, 24, w, h, "mixed")
+    net = dev.UNet(1, h, w, seed=7)
+    r = dev.Renderer(w, h, 4, 0, 6, lgx.SAMPLER_HIZ_POINT, 8, kind="mixed", denoiser=lgx.DENOISER_UNET,
+                     profile=True, footprint_dirs=256)
+    r.attach_unet(net)
+    r.render(g, view, proj)
+    torch.cuda.synchronize()
+    eager = r.output()
+    s = torch.cuda.Stream()
+    r.capture(g, view, proj, stream=s)
+    for i in range(3):
+        junk = torch.randn(1 << 20, device="cuda") * 2.0
+        torch.cuda.synchronize()
+        r.replay(stream=s)
+        torch.cuda.synchronize()
+        assert r.output().tobytes() == eager.tobytes(), f"replay {i}"
+        del junk
+
+
 def test_regression_and_composite(dev):
     w, h = 256, 128
     view, proj, gb, g = frame_inputs(dev, 23, w, h, "f32")

@@ -70,6 +70,8 @@ def parse():
                     help="time the denoised path only (no error sweep against the FP64 target)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch a 256 MiB device copy 3x and the 1080p gather 3x (for rocprofv3 --pmc)")
+    ap.add_argument("--probe-denoised", action="store_true",
+                    help="with --traffic-probe: the denoised path's gather configuration")
     ap.add_argument("--unet-probe", action="store_true",
                     help="only run the 1080p U-Net denoiser pass 3x and 3 training steps (batch 16 of 256x256) "
                          "with autotuned tiles (for rocprofv3 --pmc)")
@@ -97,7 +99,7 @@ def main():
     view, proj = lgx.scene_matrices(scene, W, H)
     g = dev.GBufferDev.alloc(W, H, "mixed").render(scene)
     if args.traffic_probe:
-        traffic_probe(g, view, proj, sampler, args.gather_tile)
+        traffic_probe(g, view, proj, sampler, args.gather_tile, args.probe_denoised)
         return
     if args.unet_probe:
         unet_probe(args, g)
@@ -196,21 +198,36 @@ def main():
 COPY_BYTES = 256 << 20
 
 
-def traffic_probe(g, view, proj, sampler, tile=-1):
+def traffic_probe(g, view, proj, sampler, tile=-1, denoised=False):
     """Kernels for the counters: a plain device copy of known size (the
     traffic calibration) and the bench's gather launch, three times each, in
     the launch shape the bench times (`tile`, or the autotuner's pick; the
-    autotuner's own launches precede the copy)."""
+    autotuner's own launches precede the copy).  `denoised`: the denoised
+    path's gather (4 directions, 256-direction footprint, no step cap, 8x8
+    pattern) instead of the plain one; its step and hit counts are printed."""
     import torch
     from legitengine_amd import device as dev
+    from legitengine_amd import denoiser as dn
     from legitengine_amd import lgx
     pyr = dev.PyramidDev(g, LEVELS, sampler).build()
+    cfg = dict(dirs=DIRS, steps=STEPS, footprint=0, grid=GRID)
+    if denoised:
+        cfg = dict(dirs=DEN_DIRS, steps=dn.NOISY_STEPS, footprint=dn.NOISY_FOOTPRINT, grid=dn.NOISY_GRID)
     if tile < 0:
-        tile, _ = dev.autotune_gather(g, pyr, view, proj, DIRS, STEPS, sampler)
+        tile, _ = dev.autotune_gather(g, pyr, view, proj, cfg["dirs"], cfg["steps"], sampler, cfg["footprint"],
+                                      grid=cfg["grid"])
     x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
-    ga = dev.Gather(W, H, DIRS, STEPS, sampler, tile=tile)
+    ga = dev.Gather(W, H, cfg["dirs"], cfg["steps"], sampler, grid=cfg["grid"], tile=tile,
+                    footprint_dirs=cfg["footprint"])
     out = dev.DImage(W, H, lgx.RGBA32F)
+    if denoised:
+        counts = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+        hits = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+        dev.Gather(W, H, cfg["dirs"], cfg["steps"], sampler, grid=cfg["grid"],
+                   footprint_dirs=cfg["footprint"])(view, proj, g, pyr, step_counts=counts, hit_counts=hits)
+        torch.cuda.synchronize()
+        print(json.dumps({"steps_executed": int(counts.sum().item()), "hits": int(hits.sum().item())}), flush=True)
     torch.cuda.synchronize()
     for _ in range(3):
         y.copy_(x)
@@ -359,8 +376,8 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
     net.load(path)
     unet_tune = net.autotune(1, H, W)
     pyr0 = dev.PyramidDev(g, LEVELS, sampler).build()
-    den_tile, _ = dev.autotune_gather(g, pyr0, view, proj, DEN_DIRS, dn.NOISY_STEPS, sampler, dn.NOISY_FOOTPRINT,
-                                      grid=dn.NOISY_GRID)
+    den_tile, den_tiles = dev.autotune_gather(g, pyr0, view, proj, DEN_DIRS, dn.NOISY_STEPS, sampler,
+                                              dn.NOISY_FOOTPRINT, grid=dn.NOISY_GRID)
     rden = dev.Renderer(W, H, DEN_DIRS, dn.NOISY_STEPS, LEVELS, sampler, dn.NOISY_GRID, denoiser=lgx.DENOISER_UNET,
                         kind="mixed", profile=True, footprint_dirs=dn.NOISY_FOOTPRINT, tile=den_tile)
     rden.attach_unet(net)
@@ -380,6 +397,7 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         "unet_tiles": [r["best"] for r in unet_tune["forward"]],
         "unet_layer_ms": [round(min(r["ms"]), 4) for r in unet_tune["forward"]],
         "gather_tile": den_tile,
+        "gather_tile_ms": {k: round(v, 4) for k, v in den_tiles.items()},
     }
     if args.no_quality:
         return timing

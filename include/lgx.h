@@ -333,8 +333,14 @@ void lgx_unet_destroy(lgx_unet *u);
 /* Pitch-aware device-to-device copy of equal-extent, equal-format images (async). */
 int lgx_copy_image(const lgx_image *src, lgx_image *dst, lgx_stream_t stream);
 
-/* Number of gather launch shapes (lgx_gather_params.tile indexes them). */
+/* Number of gather launch shapes (lgx_gather_params.tile indexes them).  The
+ * shapes from lgx_gather_batch_tiles() up re-lay the pyramid levels the
+ * pattern's sub-images share into column-swizzled copies held in the
+ * workspace (single-view FP32 HiZ gathers only; others ignore that part). */
 int lgx_gather_tiles(void);
+/* Number of launch shapes the persistent multi-view gather accepts (the
+ * first ones of lgx_gather_tiles()). */
+int lgx_gather_batch_tiles(void);
 
 /* ---- several views of uneven resolution in one persistent launch ----
  * No counterpart in the reference (its frame loop, main.cpp:269-312, renders

@@ -308,7 +308,7 @@ extern "C" int lgx_batch_autotune(lgx_batch *b, lgx_stream_t stream, char *buf, 
     const int wg_mult[] = {0, 4, 8, 16};
     const int saved_wg = b->c.workgroups;
     try {
-      for (int t = 0; t < lgx_gather_tiles(); t++) {
+      for (int t = 0; t < lgx_gather_batch_tiles(); t++) {
         for (lgx_gather_batch *gb : b->gbs) Check(lgx_gather_batch_set_tile(gb, t), "lgx_gather_batch_set_tile");
         for (int m : wg_mult) {
           const int wg = m * cus;

@@ -56,6 +56,10 @@ template <class T> struct GatherArgs {
   const void *pr[LGX_MAX_LEVELS];
   int pd_pitch[LGX_MAX_LEVELS], pr_pitch[LGX_MAX_LEVELS];
   int lw[LGX_MAX_LEVELS], lh[LGX_MAX_LEVELS];
+  // Column-swizzled levels (SWZ kernels): texel column x of level l lives at
+  // column (x & swz_mask[l]) * swz_w[l] + (x >> swz_shift[l]) of pd / pr
+  // (mask and shift 0 on levels kept as they are)
+  int swz_mask[LGX_MAX_LEVELS], swz_shift[LGX_MAX_LEVELS], swz_w[LGX_MAX_LEVELS];
   void *out;
   int out_pitch, out_slices;
   int *step_counts;
@@ -63,6 +67,7 @@ template <class T> struct GatherArgs {
   int tile_w, tile_h, tiles_x, n_blocks;
   int occ;   // host side: which register allocation to launch
   int pipe;  // host side: which step loop to launch
+  int swz;   // host side: the fine levels are column-swizzled copies
   int sx0, sy0, lcells_x, lcells_y;  // slice cells covered by the launch (region of interest)
   int rx0, ry0, rx1, ry1;            // pixels gathered: [rx0, rx1) x [ry0, ry1)
 };
@@ -272,7 +277,7 @@ template <class T> LGX_D size_t point_index(T sx, T sy, int lw, int lh, int pitc
 // ARGS: GatherArgs<T> as the kernel's argument, or in constant memory
 // (address space 4, the multi-view kernel's per-view table): either way the
 // compiler reads its fields as scalars.
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, bool PIPE, class ARGS>
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, bool PIPE, bool SWZ, class ARGS>
 LGX_D void gather_block(ARGS &a, const uint32_t b) {
   const int g2 = a.grid * a.grid_y;
   const int k = (int)(b % (uint32_t)g2);
@@ -379,6 +384,17 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
       __amdgpu_buffer_rsrc_t drs = depth_rsrc(0), rrs = rad_rsrc(0);
       T flw = T(a.lw[0]), flh = T(a.lh[0]);
       int lw1 = a.lw[0] - 1, lh1 = a.lh[0] - 1, dpitch = a.pd_pitch[0], rpitch = a.pr_pitch[0];
+      int smask = 0, sshift = 0, ssw = 0;
+      if constexpr (SWZ) {
+        smask = a.swz_mask[0];
+        sshift = a.swz_shift[0];
+        ssw = a.swz_w[0];
+      }
+      // the storage column of texel column ix on the current level
+      auto col = [&](int ix) {
+        if constexpr (SWZ) return (int)(__umul24((unsigned)(ix & smask), (unsigned)ssw) + (unsigned)(ix >> sshift));
+        return ix;
+      };
       // the fetch of step s at offset o on the current level: on-screen flag,
       // q / extent, depth request, radiance texel index
       struct Fetch {
@@ -398,7 +414,7 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
         f.sqy = div_extent(qy, a.H, a.rH);
         // sq > 0 on screen, so sq * extent >= 0 and truncation is floor; the
         // product can round up to the extent itself, hence the upper clamp
-        const int ix = min((int)(f.sqx * flw), lw1);
+        const int ix = col(min((int)(f.sqx * flw), lw1));
         const int iy = min((int)(f.sqy * flh), lh1);
         f.rti = texel_index(iy, rpitch, ix);
         const uint32_t ti = texel_index(iy, dpitch, ix);
@@ -418,6 +434,11 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
           flh = T(a.lh[l]);
           lw1 = a.lw[l] - 1;
           lh1 = a.lh[l] - 1;
+          if constexpr (SWZ) {
+            smask = a.swz_mask[l];
+            sshift = a.swz_shift[l];
+            ssw = a.swz_w[l];
+          }
         }
       };
       if constexpr (PIPE) {
@@ -486,7 +507,7 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
           const T sqx = div_extent(qx, a.W, a.rW), sqy = div_extent(qy, a.H, a.rH);  // :219
           // sq > 0, so sq * extent >= 0 and truncation is floor; the product can
           // round up to the extent itself, hence the upper clamp (in integers)
-          const int ix = min((int)(sqx * flw), lw1);
+          const int ix = col(min((int)(sqx * flw), lw1));
           const int iy = min((int)(sqy * flh), lh1);
           const uint32_t ti = texel_index(iy, dpitch, ix);
           const T r = T(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(ti * 4u), 0, 0)));
@@ -622,9 +643,41 @@ LGX_D void gather_block(ARGS &a, const uint32_t b) {
 // OCC: minimum waves per SIMD the register allocation must allow (1 = the
 // compiler's choice); the launch shapes offer 1 and 8.  PIPE: the step loop
 // that overlaps fetches with arithmetic (Hi-Z sampler).
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, int OCC, bool PIPE = false>
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, bool COUNT, int OCC, bool PIPE = false, bool SWZ = false>
 __global__ __launch_bounds__(256, OCC) void gather_kernel(const GatherArgs<T> a) {
-  gather_block<T, A, SAMPLER, RFMT, NFMT, COUNT, PIPE>(a, xcd_remap(blockIdx.x, (uint32_t)a.n_blocks));
+  gather_block<T, A, SAMPLER, RFMT, NFMT, COUNT, PIPE, SWZ>(a, xcd_remap(blockIdx.x, (uint32_t)a.n_blocks));
+}
+
+// Column-swizzled copies of the fine pyramid levels for the SWZ kernels.
+// With a pattern of G columns, the lanes of a wave (one slice) sample points
+// G pixels apart, so on level l (G >> l >= 2) their texel columns share their
+// residue mod F = G >> l: stored as F sub-images of every F-th column
+// (column x at (x mod F) * sw + x / F), a wave's texels are contiguous.
+struct SwzJob {
+  const void *src;
+  void *dst;
+  int src_pitch, dst_pitch, w, h, mask, shift, sw, bytes;  // pitches in texels; bytes per texel: 4, 8 or 16
+};
+struct SwzJobs {
+  SwzJob j[2 * LGX_MAX_LEVELS];
+  int n;
+};
+// One launch for all jobs: blockIdx.y is the job, workgroups stride over its
+// rows, the threads over a row's columns.
+__global__ __launch_bounds__(256) void swizzle_levels_kernel(const SwzJobs jobs) {
+  const SwzJob &j = jobs.j[blockIdx.y];
+  for (int y = blockIdx.x; y < j.h; y += gridDim.x) {
+    const size_t srow = (size_t)y * j.src_pitch, drow = (size_t)y * j.dst_pitch;
+    for (int x = threadIdx.x; x < j.w; x += 256) {
+      const size_t si = srow + x, di = drow + (x & j.mask) * j.sw + (x >> j.shift);
+      if (j.bytes == 4)
+        ((uint32_t *)j.dst)[di] = ((const uint32_t *)j.src)[si];
+      else if (j.bytes == 8)
+        ((uint2 *)j.dst)[di] = ((const uint2 *)j.src)[si];
+      else
+        ((uint4 *)j.dst)[di] = ((const uint4 *)j.src)[si];
+    }
+  }
 }
 
 // Persistent gather over several views: the logical blocks of all views form
@@ -661,7 +714,7 @@ __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs
       __syncthreads();  // every wave has read s_item before lane 0 overwrites it
       if (it >= c1) break;
       while (v + 1 < n_views && it >= first[v + 1]) v++;  // items of one chunk only grow
-      gather_block<T, A, SAMPLER, RFMT, NFMT, false, PIPE>(cviews[v], it - first[v]);
+      gather_block<T, A, SAMPLER, RFMT, NFMT, false, PIPE, false>(cviews[v], it - first[v]);
     }
   }
 }
@@ -723,11 +776,13 @@ static GatherTableShape table_shape(const lgx_gather_params *p) {
 // What a workspace's tables were computed for: every parameter the tables
 // depend on, normalised (grid_y <= 0 is grid, footprint <= 0 is dirs; F32 and
 // F32_ACC64 share the float tables).
+static bool uses_swizzle(const lgx_gather_params *p);
+
 struct TableKey {
-  int32_t width, height, dirs, max_steps, grid, grid_y, footprint, f64;
+  int32_t width, height, dirs, max_steps, grid, grid_y, footprint, f64, swz;
   bool operator==(const TableKey &o) const {
     return width == o.width && height == o.height && dirs == o.dirs && max_steps == o.max_steps && grid == o.grid &&
-           grid_y == o.grid_y && footprint == o.footprint && f64 == o.f64;
+           grid_y == o.grid_y && footprint == o.footprint && f64 == o.f64 && swz == o.swz;
   }
 };
 
@@ -741,6 +796,7 @@ static TableKey table_key(const lgx_gather_params *p) {
   k.grid_y = p->grid_y > 0 ? p->grid_y : p->grid;
   k.footprint = p->footprint_dirs > 0 ? p->footprint_dirs : p->dirs;
   k.f64 = p->precision == LGX_PREC_F64;
+  k.swz = uses_swizzle(p);  // the workspace then also holds the swizzled levels
   return k;
 }
 
@@ -855,12 +911,22 @@ static hipError_t launch_gather(const GatherArgs<T> &a, bool count, hipStream_t 
   dim3 grid((unsigned)a.n_blocks), block(256);
   // the FP32 Hi-Z gather has every variant; the others run the default one
   constexpr bool kAll = sizeof(T) == 4 && sizeof(A) == 4 && SAMPLER == LGX_SAMPLER_HIZ_POINT;
+  constexpr int O8 = kAll ? 8 : 1;
   if (count)
     hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, true, 1>), grid, block, 0, s, a);
-  else if (kAll && a.occ == 8 && a.pipe)
-    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, kAll ? 8 : 1, kAll>), grid, block, 0, s, a);
+  else if (kAll && a.swz) {  // (make_args sets swz only for this gather)
+    if (a.occ == 8 && a.pipe)
+      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, O8, kAll, kAll>), grid, block, 0, s, a);
+    else if (a.occ == 8)
+      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, O8, false, kAll>), grid, block, 0, s, a);
+    else if (a.pipe)
+      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1, kAll, kAll>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1, false, kAll>), grid, block, 0, s, a);
+  } else if (kAll && a.occ == 8 && a.pipe)
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, O8, kAll>), grid, block, 0, s, a);
   else if (kAll && a.occ == 8)
-    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, kAll ? 8 : 1>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, O8>), grid, block, 0, s, a);
   else if (kAll && a.pipe)
     hipLaunchKernelGGL((gather_kernel<T, A, SAMPLER, RFMT, NFMT, false, 1, kAll>), grid, block, 0, s, a);
   else
@@ -880,23 +946,77 @@ static hipError_t dispatch_fmt(const GatherArgs<T> &a, int rfmt, int nfmt, bool 
 // Launch shapes offered to the autotuner: tile_w x tile_h slice pixels per
 // 256-thread workgroup, the occupancy the registers are allocated for (1: the
 // compiler's choice, 6 waves per SIMD; 8: 64 VGPRs with a few spilled set-up
-// values), and the step loop (1: fetches overlapped with arithmetic).  Every
-// shape computes the same arithmetic in the same order.
-static const int kTiles[][4] = {{16, 16, 1, 0}, {8, 32, 1, 0}, {32, 8, 1, 0}, {64, 4, 1, 0},
-                                {16, 16, 8, 0}, {8, 32, 8, 0}, {32, 8, 8, 0}, {64, 4, 8, 0},
-                                {16, 16, 1, 1}, {8, 32, 1, 1}, {32, 8, 1, 1}, {64, 4, 1, 1},
-                                {16, 16, 8, 1}, {8, 32, 8, 1}, {32, 8, 8, 1}, {64, 4, 8, 1}};
-static const int kNumTiles = 16;
+// values), the step loop (1: fetches overlapped with arithmetic), and the
+// pyramid layout (1: the fine levels read from column-swizzled copies made in
+// the workspace at each call; single-view FP32 Hi-Z gathers on a power-of-two
+// pattern, otherwise as 0).  Every shape computes the same arithmetic in the
+// same order on the same texels.
+static const int kTiles[][5] = {
+    {16, 16, 1, 0, 0}, {8, 32, 1, 0, 0}, {32, 8, 1, 0, 0}, {64, 4, 1, 0, 0}, {16, 16, 8, 0, 0}, {8, 32, 8, 0, 0},
+    {32, 8, 8, 0, 0},  {64, 4, 8, 0, 0}, {16, 16, 1, 1, 0}, {8, 32, 1, 1, 0}, {32, 8, 1, 1, 0}, {64, 4, 1, 1, 0},
+    {16, 16, 8, 1, 0}, {8, 32, 8, 1, 0}, {32, 8, 8, 1, 0},  {64, 4, 8, 1, 0}, {16, 16, 1, 0, 1}, {8, 32, 1, 0, 1},
+    {32, 8, 1, 0, 1},  {64, 4, 1, 0, 1}, {16, 16, 8, 0, 1}, {8, 32, 8, 0, 1}, {32, 8, 8, 0, 1}, {64, 4, 8, 0, 1},
+    {16, 16, 1, 1, 1}, {8, 32, 1, 1, 1}, {32, 8, 1, 1, 1}, {64, 4, 1, 1, 1}, {16, 16, 8, 1, 1}, {8, 32, 8, 1, 1},
+    {32, 8, 8, 1, 1},  {64, 4, 8, 1, 1}};
+static const int kNumTiles = 32;
+static const int kBatchTiles = 16;  // the persistent multi-view gather reads the pyramids as they are
+
+// Levels stored column-swizzled: those with pattern columns / 2^l >= 2.
+static int swz_levels(const lgx_gather_params *p) {
+  const int g = p->grid;
+  if (g < 2 || (g & (g - 1))) return 0;
+  int n = 0;
+  while ((g >> n) > 1 && (p->width >> n) >= 1 && (p->height >> n) >= 1) n++;
+  return std::min(n, LGX_MAX_LEVELS);
+}
+static bool uses_swizzle(const lgx_gather_params *p) {
+  return p->tile >= 0 && p->tile < kNumTiles && kTiles[p->tile][4] && p->precision == LGX_PREC_F32 &&
+         p->sampler == LGX_SAMPLER_HIZ_POINT && swz_levels(p) > 0;
+}
+struct SwzLevel {
+  int f, shift, sw, pitch, w, h;
+  size_t off_d, off_r;  // byte offsets in the workspace
+};
+// The swizzled levels after the tables (base bytes), 256-byte aligned, with
+// room for RGBA32F radiance; returns the end.
+static size_t swz_layout(const lgx_gather_params *p, size_t base, SwzLevel *lv, int *n) {
+  const int nl = swz_levels(p);
+  size_t off = (base + 255) / 256 * 256;
+  for (int l = 0; l < nl; l++) {
+    SwzLevel &L = lv[l];
+    L.f = p->grid >> l;
+    L.shift = 0;
+    while ((1 << L.shift) < L.f) L.shift++;
+    L.w = p->width >> l;  // floor halving, as the pyramid's levels
+    L.h = p->height >> l;
+    L.sw = (L.w + L.f - 1) / L.f;
+    L.pitch = L.f * L.sw;
+    L.off_d = off;
+    off += ((size_t)L.pitch * L.h * 4 + 255) / 256 * 256;
+    L.off_r = off;
+    off += ((size_t)L.pitch * L.h * 16 + 255) / 256 * 256;
+  }
+  *n = nl;
+  return off;
+}
 
 }  // namespace lgx
 
 using namespace lgx;
 
-extern "C" size_t lgx_gather_workspace_bytes(const lgx_gather_params *p) {
-  if (!gather_params_ok(p)) return 0;
+static size_t table_bytes(const lgx_gather_params *p) {
   GatherTableShape s = table_shape(p);
   return ((size_t)s.g2 * s.D * 4 + (size_t)s.g2 * s.S_tab * 2) * s.elem +
          (size_t)s.g2 * (LGX_MAX_LEVELS + 1) * sizeof(int32_t);
+}
+
+extern "C" size_t lgx_gather_workspace_bytes(const lgx_gather_params *p) {
+  if (!gather_params_ok(p)) return 0;
+  const size_t t = table_bytes(p);
+  if (!uses_swizzle(p)) return t;
+  SwzLevel lv[LGX_MAX_LEVELS];
+  int n = 0;
+  return swz_layout(p, t, lv, &n);
 }
 
 extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
@@ -912,10 +1032,12 @@ extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
 }
 
 extern "C" int lgx_gather_tiles(void) { return kNumTiles; }
+extern "C" int lgx_gather_batch_tiles(void) { return kBatchTiles; }
 
+// jobs: the swizzle copies a swizzling launch needs first (null: never swizzle)
 template <class T>
 static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
-                     lgx_image *out, GatherArgs<T> &a) {
+                     lgx_image *out, GatherArgs<T> &a, SwzJobs *jobs = nullptr) {
   GatherTableShape s = table_shape(p);
   memset(&a, 0, sizeof(a));
   frame_constants<T>(p, a);
@@ -974,6 +1096,29 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
     a.rx1 = p->roi[0] + p->roi[2];
     a.ry1 = p->roi[1] + p->roi[3];
   }
+  if (jobs) jobs->n = 0;
+  if (jobs && uses_swizzle(p) && !p->step_counts) {
+    SwzLevel lv[LGX_MAX_LEVELS];
+    int n = 0;
+    swz_layout(p, table_bytes(p), lv, &n);
+    n = std::min(n, pyr->n_levels);
+    const int rb = pyr->radiance[0].format == LGX_RGBA16F ? 8 : 16;
+    a.swz = 1;
+    for (int l = 0; l < n; l++) {
+      const SwzLevel &L = lv[l];
+      void *dd = (char *)ws + L.off_d, *dr = (char *)ws + L.off_r;
+      jobs->j[jobs->n++] = SwzJob{pyr->depth[l].ptr, dd, pyr->depth[l].pitch, L.pitch, L.w, L.h, L.f - 1, L.shift,
+                                  L.sw, 4};
+      jobs->j[jobs->n++] = SwzJob{pyr->radiance[l].ptr, dr, pyr->radiance[l].pitch, L.pitch, L.w, L.h, L.f - 1,
+                                  L.shift, L.sw, rb};
+      a.pd[l] = (const float *)dd;
+      a.pr[l] = dr;
+      a.pd_pitch[l] = a.pr_pitch[l] = L.pitch;
+      a.swz_mask[l] = L.f - 1;
+      a.swz_shift[l] = L.shift;
+      a.swz_w[l] = L.sw;
+    }
+  }
   a.sx0 = a.rx0 / a.grid;
   a.sy0 = a.ry0 / a.grid_y;
   a.lcells_x = (a.rx1 + a.grid - 1) / a.grid - a.sx0;
@@ -988,9 +1133,18 @@ template <class T, class A>
 static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
                        lgx_image *out, hipStream_t stream) {
   GatherArgs<T> a;
-  int rc = make_args<T>(p, ws, g, pyr, out, a);
+  SwzJobs jobs;
+  int rc = make_args<T>(p, ws, g, pyr, out, a, &jobs);
   if (rc != LGX_OK) return rc;
   hipError_t e;
+  if (jobs.n) {
+    int rows = 1;
+    for (int i = 0; i < jobs.n; i++) rows = std::max(rows, jobs.j[i].h);
+    const unsigned bx = (unsigned)std::min(rows, 4096);
+    hipLaunchKernelGGL(swizzle_levels_kernel, dim3(bx, (unsigned)jobs.n), dim3(256), 0, stream, jobs);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_status(e);
+  }
   const int rf = pyr->radiance[0].format, nf = g->normal.format;
   if (p->sampler == LGX_SAMPLER_HIZ_POINT)
     e = dispatch_fmt<T, A, LGX_SAMPLER_HIZ_POINT>(a, rf, nf, p->step_counts != nullptr, stream);
@@ -1125,7 +1279,7 @@ extern "C" int lgx_gather_batch_create(const lgx_gather_params *p, const void *c
                                        lgx_gather_batch **res) {
   if (!p || !workspaces || !g || !pyr || !out || !res || n < 1) return LGX_EINVAL;
   *res = nullptr;
-  if (p[0].tile < 0 || p[0].tile >= kNumTiles) return LGX_EUNSUPPORTED;
+  if (p[0].tile < 0 || p[0].tile >= kBatchTiles) return LGX_EUNSUPPORTED;
   for (int i = 0; i < n; i++) {
     int rc = check_gather(&p[i], workspaces[i], &g[i], &pyr[i], &out[i]);
     if (rc != LGX_OK) return rc;
@@ -1159,7 +1313,7 @@ extern "C" int lgx_gather_batch_create(const lgx_gather_params *p, const void *c
 
 extern "C" int lgx_gather_batch_set_tile(lgx_gather_batch *b, int32_t tile) {
   if (!b) return LGX_EINVAL;
-  if (tile < 0 || tile >= kNumTiles) return LGX_EUNSUPPORTED;
+  if (tile < 0 || tile >= kBatchTiles) return LGX_EUNSUPPORTED;
   if (tile == b->tile) return LGX_OK;
   return gather_batch_upload(b, tile);
 }
@@ -1213,6 +1367,7 @@ extern "C" int lgx_gather_noalloc(const lgx_gather_params *p, const lgx_gbuffer 
                                   lgx_image *out, lgx_stream_t stream) {
   const int rc = check_gather_images(p, g, pyr, out);
   if (rc != LGX_OK) return rc;
+  if (uses_swizzle(p)) return LGX_EUNSUPPORTED;  // the scratch array holds the tables only
   const GatherTableShape s = table_shape(p);
   if (s.S_tab > LGX_MAX_STEPS) return LGX_EUNSUPPORTED;
   if (lgx_gather_workspace_bytes(p) > (size_t)LGX_GATHER_SCRATCH_BYTES) return LGX_EUNSUPPORTED;

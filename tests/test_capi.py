@@ -115,8 +115,12 @@ def test_batch_argument_checks():
     pyr.radiance[0] = lgx.image(fake, w, hh, lgx.RGBA16F)
     out = lgx.image(fake, w, hh, lgx.RGBA32F)
     ws = (C.c_void_p * 1)(fake)
+    # shapes from lgx_gather_batch_tiles() up re-lay the pyramid in the
+    # workspace per call: single-view only
+    nb = L.lgx_gather_batch_tiles()
+    assert 0 < nb < L.lgx_gather_tiles()
     for field, value, code in (("precision", lgx.PREC_F64, lgx.EFORMAT), ("tile", 99, lgx.EUNSUPPORTED),
-                               ("dirs", 4, lgx.EWORKSPACE)):
+                               ("tile", nb, lgx.EUNSUPPORTED), ("dirs", 4, lgx.EWORKSPACE)):
         q = lgx.GatherParams()
         C.memmove(C.byref(q), C.byref(p), C.sizeof(q))
         setattr(q, field, value)
@@ -124,6 +128,36 @@ def test_batch_argument_checks():
         assert rc == code and not h.value, (field, rc)
     # a workspace lgx_gather_prepare never filled is rejected before any launch
     assert L.lgx_gather(C.byref(p), C.c_void_p(fake), C.byref(g), C.byref(pyr), C.byref(out), None) == lgx.EWORKSPACE
+    # the allocation-free gather keeps only tables in its array
+    q = lgx.GatherParams()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(q))
+    q.tile = nb
+    assert L.lgx_gather_noalloc(C.byref(q), C.byref(g), C.byref(pyr), C.byref(out), None) == lgx.EUNSUPPORTED
+
+
+def test_swizzled_workspace_size():
+    """A launch shape that swizzles adds room for copies of the levels the
+    pattern's columns share (pattern 8: levels 0-2, 4 + 16 bytes a texel);
+    others, and samplers or precisions that never swizzle, do not."""
+    import ctypes as C
+    L = lgx.lib()
+    nb = L.lgx_gather_batch_tiles()
+    p = lgx.GatherParams()
+    p.width, p.height, p.dirs, p.max_steps, p.grid, p.footprint_dirs = 1920, 1080, 4, 0, 8, 256
+    p.sampler = lgx.SAMPLER_HIZ_POINT
+    base = L.lgx_gather_workspace_bytes(C.byref(p))
+    texels = sum((1920 >> l) * (1080 >> l) for l in range(3))
+    for t in range(nb, L.lgx_gather_tiles()):
+        p.tile = t
+        extra = L.lgx_gather_workspace_bytes(C.byref(p)) - base
+        assert 20 * texels <= extra <= 20 * texels + 7 * 256
+    for field, value in (("sampler", lgx.SAMPLER_TRILINEAR), ("precision", lgx.PREC_F32_ACC64), ("grid", 1)):
+        q = lgx.GatherParams()
+        C.memmove(C.byref(q), C.byref(p), C.sizeof(q))
+        setattr(q, field, value)
+        b0 = L.lgx_gather_workspace_bytes(C.byref(q))
+        q.tile = 0
+        assert b0 == L.lgx_gather_workspace_bytes(C.byref(q)), field
 
 
 @pytest.fixture(scope="module")

@@ -54,7 +54,7 @@ def test_gather_batch_matches_single_view(dev, views, footprint):
     gb = dev.GatherBatch(items)
     expect_blocks = sum(((w // 4 + 15) // 16) * ((h // 4 + 15) // 16) * 16 for (w, h) in SIZES)
     assert gb.blocks == expect_blocks
-    for tile in range(lgx.lib().lgx_gather_tiles()):
+    for tile in range(lgx.lib().lgx_gather_batch_tiles()):
         gb.set_tile(tile)
         refs = [single(dev, g, v, p, tile, footprint=footprint) for (g, v, p) in views]
         for wg in (0, 8, 64, 1000):
@@ -104,7 +104,7 @@ def test_batch_matches_pieces_and_orders_latencies(dev, views, net):
 def test_batch_capture_replay_and_autotune(dev, views, net):
     b = dev.Batch(views, dirs=4, max_steps=64, levels=6, net=net)
     rep = b.autotune()
-    assert len(rep["table"]) == 4 * lgx.lib().lgx_gather_tiles()
+    assert len(rep["table"]) == 4 * lgx.lib().lgx_gather_batch_tiles()
     assert b.shape() == (rep["best"]["tile"], rep["best"]["workgroups"])
     b.run()
     torch.cuda.synchronize()

@@ -227,13 +227,46 @@ def test_gather_slices_layout(dev, c1):
     assert back.numpy().tobytes() == a.numpy().tobytes()
 
 
-@pytest.mark.parametrize("tile", list(range(16)))  # every launch shape lgx_gather_tiles() offers
+@pytest.mark.parametrize("tile", list(range(32)))  # every launch shape lgx_gather_tiles() offers
 def test_gather_tiles_identical(dev, c1, tile):
     sc, view, proj, gb = c1[1]
     g = dev.GBufferDev.from_host(gb)
     pyr = dev.PyramidDev(g, 5, lgx.SAMPLER_HIZ_POINT).build()
     a = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT, tile=0)(view, proj, g, pyr)
     b = dev.Gather(g.w, g.h, 16, 32, lgx.SAMPLER_HIZ_POINT, tile=tile)(view, proj, g, pyr)
+    torch.cuda.synchronize()
+    assert a.numpy().tobytes() == b.numpy().tobytes()
+
+
+@pytest.mark.parametrize("grid,w,h,roi", [(8, 330, 190, None), ((4, 16), 322, 200, None), ((16, 4), 320, 96, None),
+                                          (8, 256, 144, (40, 24, 130, 70)), (2, 97, 61, None)])
+@pytest.mark.parametrize("tile", [16, 20, 24, 28])
+def test_gather_swizzled_levels_identical(dev, grid, w, h, roi, tile):
+    """The launch shapes from lgx_gather_batch_tiles() up read column-swizzled
+    copies of the fine levels (RGBA16F radiance here, widths that are not
+    multiples of the pattern, a region of interest, two gathers in a row on
+    one workspace): the same bits as the plain layout."""
+    assert tile >= lgx.lib().lgx_gather_batch_tiles()
+    sc, view, proj, gb = host_scene(11, w, h, "mixed", 8, 2)
+    g = dev.GBufferDev.from_host(gb)
+    pyr = dev.PyramidDev(g, 6, lgx.SAMPLER_HIZ_POINT).build()
+    ref = dev.Gather(w, h, 4, 0, lgx.SAMPLER_HIZ_POINT, grid=grid, footprint_dirs=256, tile=0)
+    swz = dev.Gather(w, h, 4, 0, lgx.SAMPLER_HIZ_POINT, grid=grid, footprint_dirs=256, tile=tile)
+    assert swz.ws.numel() > ref.ws.numel()
+    oa, ob = dev.DImage(w, h, lgx.RGBA32F), dev.DImage(w, h, lgx.RGBA32F)
+    for _ in range(2):
+        oa.t.fill_(-1.0)
+        ob.t.fill_(-1.0)
+        ref(view, proj, g, pyr, out=oa, roi=roi)
+        swz(view, proj, g, pyr, out=ob, roi=roi)
+        torch.cuda.synchronize()
+        assert oa.numpy().tobytes() == ob.numpy().tobytes()
+    # a pyramid built from another frame: the copies are made per call
+    sc2, view2, proj2, gb2 = host_scene(12, w, h, "mixed", 8, 2)
+    g2 = dev.GBufferDev.from_host(gb2)
+    pyr2 = dev.PyramidDev(g2, 6, lgx.SAMPLER_HIZ_POINT).build()
+    a = ref(view2, proj2, g2, pyr2)
+    b = swz(view2, proj2, g2, pyr2)
     torch.cuda.synchronize()
     assert a.numpy().tobytes() == b.numpy().tobytes()
 

@@ -133,8 +133,11 @@ size_t lgx_gather_workspace_bytes(const lgx_gather_params *p);
 int lgx_gather_prepare(const lgx_gather_params *p, void *workspace);
 
 /* IndirectLightPass (SSVGIRenderer.h:223-263; indirectLighting.frag:114-272).
- * out: RGBA32F (F32) or RGBA64F (F64), width x height. */
-int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+ * out: RGBA32F (F32) or RGBA64F (F64), width x height.  A launch shape that
+ * swizzles (lgx_gather_tile_info) first writes column-swizzled copies of the
+ * fine pyramid levels into the workspace, after the tables: calls that
+ * overlap in time on one such workspace race. */
+int lgx_gather(const lgx_gather_params *p, void *workspace, const lgx_gbuffer *g, const lgx_pyramid *pyr,
                lgx_image *out, lgx_stream_t stream);
 
 /* IndirectLightPass without allocations: no counterpart in the reference.
@@ -333,14 +336,17 @@ void lgx_unet_destroy(lgx_unet *u);
 /* Pitch-aware device-to-device copy of equal-extent, equal-format images (async). */
 int lgx_copy_image(const lgx_image *src, lgx_image *dst, lgx_stream_t stream);
 
-/* Number of gather launch shapes (lgx_gather_params.tile indexes them).  The
- * shapes from lgx_gather_batch_tiles() up re-lay the pyramid levels the
- * pattern's sub-images share into column-swizzled copies held in the
- * workspace (single-view FP32 HiZ gathers only; others ignore that part). */
+/* Number of gather launch shapes (lgx_gather_params.tile indexes them). */
 int lgx_gather_tiles(void);
-/* Number of launch shapes the persistent multi-view gather accepts (the
- * first ones of lgx_gather_tiles()). */
-int lgx_gather_batch_tiles(void);
+/* A launch shape: info[0..4] = slice pixels per workgroup in x and y, waves
+ * per SIMD the registers are held to (1 = the compiler's choice), pipelined
+ * step loop (0/1), swizzled levels (0/1).  A swizzling shape re-lays the
+ * pyramid levels whose texel columns the lanes of a wave share modulo
+ * F = grid >> level (F >= 2, power-of-two grid) into column-swizzled copies
+ * before the gather (FP32 Hi-Z gathers; step counting and the other
+ * samplers and precisions read the levels as they are).  All shapes give
+ * identical results. */
+int lgx_gather_tile_info(int32_t tile, int32_t *info);
 
 /* ---- several views of uneven resolution in one persistent launch ----
  * No counterpart in the reference (its frame loop, main.cpp:269-312, renders
@@ -355,7 +361,9 @@ int lgx_gather_batch_tiles(void);
 typedef struct lgx_gather_batch lgx_gather_batch;
 int lgx_gather_batch_create(const lgx_gather_params *p, const void *const *workspaces, const lgx_gbuffer *g,
                             const lgx_pyramid *pyr, lgx_image *out, int32_t n, lgx_gather_batch **res);
-/* Launch shape for every view (synchronous upload; not during a capture). */
+/* Launch shape for every view (synchronous upload; not during a capture).
+ * The first swizzling shape allocates the views' swizzled copies (held by
+ * the batch until it is destroyed); the workspaces only hold tables. */
 int lgx_gather_batch_set_tile(lgx_gather_batch *b, int32_t tile);
 /* Logical blocks (tiles x slices) of all views together. */
 int64_t lgx_gather_batch_blocks(const lgx_gather_batch *b);

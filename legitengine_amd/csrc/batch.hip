@@ -23,6 +23,7 @@
 
 #include "lgx.h"
 #include "lgx/render_graph.h"
+#include "lgx_internal.h"
 
 namespace lgx {
 
@@ -203,6 +204,7 @@ extern "C" int lgx_batch_create(const lgx_batch_config *c, const lgx_batch_view 
       auto key = std::make_pair(bv.w, bv.h);
       if (!b->ws.count(key)) {
         lgx_gather_params p = b->params(bv);
+        p.tile = gather_unswizzled_tile(p.tile);  // tables only: the gather batch holds the swizzled copies
         const size_t bytes = lgx_gather_workspace_bytes(&p);
         if (!bytes) throw Error(LGX_EINVAL, "lgx_batch_create: gather parameters");
         void *w = b->alloc(bytes);
@@ -308,7 +310,7 @@ extern "C" int lgx_batch_autotune(lgx_batch *b, lgx_stream_t stream, char *buf, 
     const int wg_mult[] = {0, 4, 8, 16};
     const int saved_wg = b->c.workgroups;
     try {
-      for (int t = 0; t < lgx_gather_batch_tiles(); t++) {
+      for (int t = 0; t < lgx_gather_tiles(); t++) {
         for (lgx_gather_batch *gb : b->gbs) Check(lgx_gather_batch_set_tile(gb, t), "lgx_gather_batch_set_tile");
         for (int m : wg_mult) {
           const int wg = m * cus;

@@ -664,8 +664,7 @@ struct SwzJobs {
 };
 // One launch for all jobs: blockIdx.y is the job, workgroups stride over its
 // rows, the threads over a row's columns.
-__global__ __launch_bounds__(256) void swizzle_levels_kernel(const SwzJobs jobs) {
-  const SwzJob &j = jobs.j[blockIdx.y];
+LGX_D void swizzle_job(const SwzJob &j) {
   for (int y = blockIdx.x; y < j.h; y += gridDim.x) {
     const size_t srow = (size_t)y * j.src_pitch, drow = (size_t)y * j.dst_pitch;
     for (int x = threadIdx.x; x < j.w; x += 256) {
@@ -679,6 +678,11 @@ __global__ __launch_bounds__(256) void swizzle_levels_kernel(const SwzJobs jobs)
     }
   }
 }
+__global__ __launch_bounds__(256) void swizzle_levels_kernel(const SwzJobs jobs) { swizzle_job(jobs.j[blockIdx.y]); }
+// the multi-view gather's jobs, in device memory
+__global__ __launch_bounds__(256) void swizzle_jobs_kernel(const SwzJob *__restrict__ jobs) {
+  swizzle_job(jobs[blockIdx.y]);
+}
 
 // Persistent gather over several views: the logical blocks of all views form
 // one list (view-major, first[v] = index of view v's first block, first[V] =
@@ -690,7 +694,7 @@ __global__ __launch_bounds__(256) void swizzle_levels_kernel(const SwzJobs jobs)
 // workgroups idle at the tail.  A workgroup whose own chunk is exhausted
 // takes blocks from the other XCDs' chunks in turn (the counters only grow,
 // so every block is taken once), and ends when all eight are exhausted.
-template <class T, class A, int SAMPLER, int RFMT, int NFMT, int OCC, bool PIPE>
+template <class T, class A, int SAMPLER, int RFMT, int NFMT, int OCC, bool PIPE, bool SWZ>
 __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs<T> *__restrict__ views,
                                                            const uint32_t *__restrict__ first, int n_views,
                                                            uint32_t *next) {
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(256, OCC) void gather_multi_kernel(const GatherArgs
       __syncthreads();  // every wave has read s_item before lane 0 overwrites it
       if (it >= c1) break;
       while (v + 1 < n_views && it >= first[v + 1]) v++;  // items of one chunk only grow
-      gather_block<T, A, SAMPLER, RFMT, NFMT, false, PIPE, false>(cviews[v], it - first[v]);
+      gather_block<T, A, SAMPLER, RFMT, NFMT, false, PIPE, SWZ>(cviews[v], it - first[v]);
     }
   }
 }
@@ -802,7 +806,9 @@ static TableKey table_key(const lgx_gather_params *p) {
 
 // Workspaces filled by lgx_gather_prepare, by device address.  lgx_gather
 // and lgx_gather_batch_create reject a workspace that was not prepared for
-// the call's parameters (a smaller table would be read past its end).
+// the call's parameters (a smaller table would be read past its end); one
+// prepared for a swizzling shape also serves the others (same tables, more
+// room).
 static std::mutex g_ws_mu;
 static std::unordered_map<const void *, TableKey> g_ws;
 
@@ -814,7 +820,10 @@ static void record_workspace(const void *ws, const lgx_gather_params *p) {
 static bool workspace_prepared(const void *ws, const lgx_gather_params *p) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   auto it = g_ws.find(ws);
-  return it != g_ws.end() && it->second == table_key(p);
+  if (it == g_ws.end()) return false;
+  TableKey need = table_key(p);
+  if (it->second.swz) need.swz = 1;
+  return it->second == need;
 }
 
 // Tables in the shader's own arithmetic and evaluation order
@@ -951,15 +960,25 @@ static hipError_t dispatch_fmt(const GatherArgs<T> &a, int rfmt, int nfmt, bool 
 // the workspace at each call; single-view FP32 Hi-Z gathers on a power-of-two
 // pattern, otherwise as 0).  Every shape computes the same arithmetic in the
 // same order on the same texels.
-static const int kTiles[][5] = {
+static constexpr int kTiles[][5] = {
     {16, 16, 1, 0, 0}, {8, 32, 1, 0, 0}, {32, 8, 1, 0, 0}, {64, 4, 1, 0, 0}, {16, 16, 8, 0, 0}, {8, 32, 8, 0, 0},
     {32, 8, 8, 0, 0},  {64, 4, 8, 0, 0}, {16, 16, 1, 1, 0}, {8, 32, 1, 1, 0}, {32, 8, 1, 1, 0}, {64, 4, 1, 1, 0},
     {16, 16, 8, 1, 0}, {8, 32, 8, 1, 0}, {32, 8, 8, 1, 0},  {64, 4, 8, 1, 0}, {16, 16, 1, 0, 1}, {8, 32, 1, 0, 1},
     {32, 8, 1, 0, 1},  {64, 4, 1, 0, 1}, {16, 16, 8, 0, 1}, {8, 32, 8, 0, 1}, {32, 8, 8, 0, 1}, {64, 4, 8, 0, 1},
     {16, 16, 1, 1, 1}, {8, 32, 1, 1, 1}, {32, 8, 1, 1, 1}, {64, 4, 1, 1, 1}, {16, 16, 8, 1, 1}, {8, 32, 8, 1, 1},
     {32, 8, 8, 1, 1},  {64, 4, 8, 1, 1}};
-static const int kNumTiles = 32;
-static const int kBatchTiles = 16;  // the persistent multi-view gather reads the pyramids as they are
+static constexpr int kNumTiles = 32;
+static constexpr int kSwzTiles = 16;  // shapes kSwzTiles + t are shape t reading swizzled levels
+constexpr bool swz_twins_ok() {
+  for (int t = 0; t < kSwzTiles; t++) {
+    for (int i = 0; i < 4; i++)
+      if (kTiles[t][i] != kTiles[t + kSwzTiles][i]) return false;
+    if (kTiles[t][4] || !kTiles[t + kSwzTiles][4]) return false;
+  }
+  return kNumTiles == 2 * kSwzTiles && sizeof(kTiles) / sizeof(kTiles[0]) == kNumTiles;
+}
+static_assert(swz_twins_ok(), "launch shape table: the second half swizzles the first");
+int gather_unswizzled_tile(int tile) { return tile >= kSwzTiles && tile < kNumTiles ? tile - kSwzTiles : tile; }
 
 // Levels stored column-swizzled: those with pattern columns / 2^l >= 2.
 static int swz_levels(const lgx_gather_params *p) {
@@ -973,15 +992,16 @@ static bool uses_swizzle(const lgx_gather_params *p) {
   return p->tile >= 0 && p->tile < kNumTiles && kTiles[p->tile][4] && p->precision == LGX_PREC_F32 &&
          p->sampler == LGX_SAMPLER_HIZ_POINT && swz_levels(p) > 0;
 }
+// The pyramid levels copied column-swizzled by a call with these parameters
+// are laid out from offset 0 of its copy area, each 256-byte aligned, with
+// room for RGBA32F radiance; returns the area's size.
 struct SwzLevel {
   int f, shift, sw, pitch, w, h;
   size_t off_d, off_r;  // byte offsets in the workspace
 };
-// The swizzled levels after the tables (base bytes), 256-byte aligned, with
-// room for RGBA32F radiance; returns the end.
-static size_t swz_layout(const lgx_gather_params *p, size_t base, SwzLevel *lv, int *n) {
+static size_t swz_layout(const lgx_gather_params *p, SwzLevel *lv, int *n) {
   const int nl = swz_levels(p);
-  size_t off = (base + 255) / 256 * 256;
+  size_t off = 0;
   for (int l = 0; l < nl; l++) {
     SwzLevel &L = lv[l];
     L.f = p->grid >> l;
@@ -1009,14 +1029,15 @@ static size_t table_bytes(const lgx_gather_params *p) {
   return ((size_t)s.g2 * s.D * 4 + (size_t)s.g2 * s.S_tab * 2) * s.elem +
          (size_t)s.g2 * (LGX_MAX_LEVELS + 1) * sizeof(int32_t);
 }
+// single view: the copy area follows the tables in the workspace
+static size_t swz_offset(const lgx_gather_params *p) { return (table_bytes(p) + 255) / 256 * 256; }
 
 extern "C" size_t lgx_gather_workspace_bytes(const lgx_gather_params *p) {
   if (!gather_params_ok(p)) return 0;
-  const size_t t = table_bytes(p);
-  if (!uses_swizzle(p)) return t;
+  if (!uses_swizzle(p)) return table_bytes(p);
   SwzLevel lv[LGX_MAX_LEVELS];
   int n = 0;
-  return swz_layout(p, t, lv, &n);
+  return swz_offset(p) + swz_layout(p, lv, &n);
 }
 
 extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
@@ -1032,12 +1053,17 @@ extern "C" int lgx_gather_prepare(const lgx_gather_params *p, void *workspace) {
 }
 
 extern "C" int lgx_gather_tiles(void) { return kNumTiles; }
-extern "C" int lgx_gather_batch_tiles(void) { return kBatchTiles; }
+extern "C" int lgx_gather_tile_info(int32_t tile, int32_t *info) {
+  if (tile < 0 || tile >= kNumTiles || !info) return LGX_EINVAL;
+  for (int i = 0; i < 5; i++) info[i] = kTiles[tile][i];
+  return LGX_OK;
+}
 
-// jobs: the swizzle copies a swizzling launch needs first (null: never swizzle)
+// jobs: the swizzle copies a swizzling launch needs first, into swz_area
+// (swz_layout's size; null: never swizzle)
 template <class T>
 static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
-                     lgx_image *out, GatherArgs<T> &a, SwzJobs *jobs = nullptr) {
+                     lgx_image *out, GatherArgs<T> &a, SwzJobs *jobs = nullptr, void *swz_area = nullptr) {
   GatherTableShape s = table_shape(p);
   memset(&a, 0, sizeof(a));
   frame_constants<T>(p, a);
@@ -1097,16 +1123,16 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
     a.ry1 = p->roi[1] + p->roi[3];
   }
   if (jobs) jobs->n = 0;
-  if (jobs && uses_swizzle(p) && !p->step_counts) {
+  if (jobs && swz_area && uses_swizzle(p) && !p->step_counts) {
     SwzLevel lv[LGX_MAX_LEVELS];
     int n = 0;
-    swz_layout(p, table_bytes(p), lv, &n);
+    swz_layout(p, lv, &n);
     n = std::min(n, pyr->n_levels);
     const int rb = pyr->radiance[0].format == LGX_RGBA16F ? 8 : 16;
     a.swz = 1;
     for (int l = 0; l < n; l++) {
       const SwzLevel &L = lv[l];
-      void *dd = (char *)ws + L.off_d, *dr = (char *)ws + L.off_r;
+      void *dd = (char *)swz_area + L.off_d, *dr = (char *)swz_area + L.off_r;
       jobs->j[jobs->n++] = SwzJob{pyr->depth[l].ptr, dd, pyr->depth[l].pitch, L.pitch, L.w, L.h, L.f - 1, L.shift,
                                   L.sw, 4};
       jobs->j[jobs->n++] = SwzJob{pyr->radiance[l].ptr, dr, pyr->radiance[l].pitch, L.pitch, L.w, L.h, L.f - 1,
@@ -1130,11 +1156,11 @@ static int make_args(const lgx_gather_params *p, const void *ws, const lgx_gbuff
 }
 
 template <class T, class A>
-static int gather_impl(const lgx_gather_params *p, const void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
+static int gather_impl(const lgx_gather_params *p, void *ws, const lgx_gbuffer *g, const lgx_pyramid *pyr,
                        lgx_image *out, hipStream_t stream) {
   GatherArgs<T> a;
   SwzJobs jobs;
-  int rc = make_args<T>(p, ws, g, pyr, out, a, &jobs);
+  int rc = make_args<T>(p, ws, g, pyr, out, a, &jobs, (char *)ws + swz_offset(p));
   if (rc != LGX_OK) return rc;
   hipError_t e;
   if (jobs.n) {
@@ -1195,7 +1221,7 @@ static int check_gather(const lgx_gather_params *p, const void *workspace, const
   return workspace_prepared(workspace, p) ? LGX_OK : LGX_EWORKSPACE;
 }
 
-extern "C" int lgx_gather(const lgx_gather_params *p, const void *workspace, const lgx_gbuffer *g,
+extern "C" int lgx_gather(const lgx_gather_params *p, void *workspace, const lgx_gbuffer *g,
                           const lgx_pyramid *pyr, lgx_image *out, lgx_stream_t stream) {
   const int rc = check_gather(p, workspace, g, pyr, out);
   if (rc != LGX_OK) return rc;
@@ -1216,8 +1242,14 @@ struct lgx_gather_batch {
   int sampler = 0, rf = 0, nf = 0, tile = 0;
   uint32_t total = 0;
   void *dev = nullptr;  // GatherArgs<float>[n], uint32 first[n + 1], uint32 next[8] (work counters)
+  // swizzling shapes: the views' swizzle jobs, then their copy areas
+  // (allocated by the first set_tile to such a shape)
+  void *swz = nullptr;
+  size_t swz_bytes = 0;
+  int njobs = 0, job_rows = 0;
   ~lgx_gather_batch() {
     if (dev) (void)hipFree(dev);
+    if (swz) (void)hipFree(swz);
   }
 };
 
@@ -1227,16 +1259,54 @@ static int gather_batch_upload(lgx_gather_batch *b, int tile) {
   const int n = (int)b->p.size();
   std::vector<GatherArgs<float>> args((size_t)n);
   std::vector<uint32_t> first((size_t)n + 1, 0);
+  // copy areas of the views (one each: views may share a workspace), after
+  // room for every view's jobs
+  std::vector<size_t> area((size_t)n, 0);
+  size_t off = ((size_t)n * 2 * LGX_MAX_LEVELS * sizeof(SwzJob) + 255) / 256 * 256;
+  bool swz = false;
   for (int i = 0; i < n; i++) {
     lgx_gather_params q = b->p[i];
     q.tile = tile;
-    int rc = make_args<float>(&q, b->ws[i], &b->g[i], &b->pyr[i], &b->out[i], args[i]);
-    if (rc != LGX_OK) return rc;
-    first[i + 1] = first[i] + (uint32_t)args[i].n_blocks;
+    if (!uses_swizzle(&q)) continue;
+    SwzLevel lv[LGX_MAX_LEVELS];
+    int nl = 0;
+    area[i] = off;
+    off += swz_layout(&q, lv, &nl);
+    swz = true;
   }
-  // no launch of this batch may still read the previous arguments
+  // no launch of this batch may still read the previous arguments or copies
   int rc0 = hip_status(hipDeviceSynchronize());
   if (rc0 != LGX_OK) return rc0;
+  if (swz && off > b->swz_bytes) {  // the same for every swizzling shape: allocated once
+    if (b->swz) (void)hipFree(b->swz);
+    b->swz_bytes = 0;
+    rc0 = hip_status(hipMalloc(&b->swz, off));
+    if (rc0 != LGX_OK) {
+      b->swz = nullptr;
+      return rc0;
+    }
+    b->swz_bytes = off;
+  }
+  std::vector<SwzJob> jobs;
+  int rows = 1;
+  for (int i = 0; i < n; i++) {
+    lgx_gather_params q = b->p[i];
+    q.tile = tile;
+    SwzJobs vj;
+    int rc = make_args<float>(&q, b->ws[i], &b->g[i], &b->pyr[i], &b->out[i], args[i], swz ? &vj : nullptr,
+                              swz ? (char *)b->swz + area[i] : nullptr);
+    if (rc != LGX_OK) return rc;
+    if (swz)
+      for (int k = 0; k < vj.n; k++) {
+        jobs.push_back(vj.j[k]);
+        rows = std::max(rows, vj.j[k].h);
+      }
+    first[i + 1] = first[i] + (uint32_t)args[i].n_blocks;
+  }
+  if (!jobs.empty()) {
+    rc0 = hip_status(hipMemcpy(b->swz, jobs.data(), jobs.size() * sizeof(SwzJob), hipMemcpyHostToDevice));
+    if (rc0 != LGX_OK) return rc0;
+  }
   const size_t ab = args.size() * sizeof(GatherArgs<float>);
   std::vector<unsigned char> blob(ab + first.size() * sizeof(uint32_t));
   memcpy(blob.data(), args.data(), ab);
@@ -1245,6 +1315,8 @@ static int gather_batch_upload(lgx_gather_batch *b, int tile) {
   if (rc != LGX_OK) return rc;
   b->tile = tile;
   b->total = first[n];
+  b->njobs = (int)jobs.size();
+  b->job_rows = std::min(rows, 4096);
   return LGX_OK;
 }
 
@@ -1257,18 +1329,35 @@ static hipError_t launch_multi(const lgx_gather_batch *b, unsigned grid, hipStre
   uint32_t *next = (uint32_t *)(first + n + 1);
   // a kernel, not a memset, so the launch can be captured into a graph
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, next, 8);
-  if (occ8 && pipe)
-    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, true>), dim3(grid), dim3(256), 0, s,
-                       args, first, n, next);
+  // the Hi-Z sampler has the swizzled variants (make_args sets no jobs otherwise)
+  constexpr bool kSwz = SAMPLER == LGX_SAMPLER_HIZ_POINT;
+  if (kSwz && b->njobs) {
+    hipLaunchKernelGGL(swizzle_jobs_kernel, dim3((unsigned)b->job_rows, (unsigned)b->njobs), dim3(256), 0, s,
+                       (const SwzJob *)b->swz);
+    if (occ8 && pipe)
+      hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, true, kSwz>), dim3(grid), dim3(256),
+                         0, s, args, first, n, next);
+    else if (occ8)
+      hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, false, kSwz>), dim3(grid),
+                         dim3(256), 0, s, args, first, n, next);
+    else if (pipe)
+      hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, true, kSwz>), dim3(grid), dim3(256),
+                         0, s, args, first, n, next);
+    else
+      hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, false, kSwz>), dim3(grid),
+                         dim3(256), 0, s, args, first, n, next);
+  } else if (occ8 && pipe)
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, true, false>), dim3(grid), dim3(256),
+                       0, s, args, first, n, next);
   else if (occ8)
-    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, false>), dim3(grid), dim3(256), 0, s,
-                       args, first, n, next);
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 8, false, false>), dim3(grid),
+                       dim3(256), 0, s, args, first, n, next);
   else if (pipe)
-    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, true>), dim3(grid), dim3(256), 0, s,
-                       args, first, n, next);
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, true, false>), dim3(grid), dim3(256),
+                       0, s, args, first, n, next);
   else
-    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, false>), dim3(grid), dim3(256), 0, s,
-                       args, first, n, next);
+    hipLaunchKernelGGL((gather_multi_kernel<float, float, SAMPLER, RFMT, NFMT, 1, false, false>), dim3(grid),
+                       dim3(256), 0, s, args, first, n, next);
   return hipGetLastError();
 }
 
@@ -1279,9 +1368,14 @@ extern "C" int lgx_gather_batch_create(const lgx_gather_params *p, const void *c
                                        lgx_gather_batch **res) {
   if (!p || !workspaces || !g || !pyr || !out || !res || n < 1) return LGX_EINVAL;
   *res = nullptr;
-  if (p[0].tile < 0 || p[0].tile >= kBatchTiles) return LGX_EUNSUPPORTED;
+  if (p[0].tile < 0 || p[0].tile >= kNumTiles) return LGX_EUNSUPPORTED;
+  if ((size_t)n * 2 * LGX_MAX_LEVELS > 65535) return LGX_EUNSUPPORTED;  // swizzle jobs on the grid's y
   for (int i = 0; i < n; i++) {
-    int rc = check_gather(&p[i], workspaces[i], &g[i], &pyr[i], &out[i]);
+    // the batch holds its views' swizzled copies itself: the workspaces
+    // need the tables only
+    lgx_gather_params q = p[i];
+    q.tile = gather_unswizzled_tile(q.tile);
+    int rc = check_gather(&q, workspaces[i], &g[i], &pyr[i], &out[i]);
     if (rc != LGX_OK) return rc;
     if (p[i].precision != LGX_PREC_F32 || p[i].step_counts || p[i].hit_counts) return LGX_EUNSUPPORTED;
     if (p[i].sampler != p[0].sampler || p[i].tile != p[0].tile ||
@@ -1313,7 +1407,7 @@ extern "C" int lgx_gather_batch_create(const lgx_gather_params *p, const void *c
 
 extern "C" int lgx_gather_batch_set_tile(lgx_gather_batch *b, int32_t tile) {
   if (!b) return LGX_EINVAL;
-  if (tile < 0 || tile >= kBatchTiles) return LGX_EUNSUPPORTED;
+  if (tile < 0 || tile >= kNumTiles) return LGX_EUNSUPPORTED;
   if (tile == b->tile) return LGX_OK;
   return gather_batch_upload(b, tile);
 }

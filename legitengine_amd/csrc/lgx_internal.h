@@ -209,6 +209,10 @@ Mat4d mat4d_mul(const Mat4d &a, const Mat4d &b);
 
 int hip_status(hipError_t e);
 
+// The gather launch shape with the same tile, registers and step loop that
+// reads the pyramid levels as they are (tile itself if it does).
+int gather_unswizzled_tile(int tile);
+
 // XCD-aware block remap: MI355X dispatches consecutive workgroups round
 // robin over its 8 XCDs (each with its own L2).  Remap so XCD x works on the
 // x-th contiguous eighth of the logical block range, keeping neighbouring

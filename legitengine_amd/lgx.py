@@ -93,7 +93,7 @@ SIGNATURES = {
     "lgx_gather": (C.c_int, [C.POINTER(GatherParams), C.c_void_p, C.POINTER(GBuffer), C.POINTER(Pyramid),
                              C.POINTER(Image), C.c_void_p]),
     "lgx_gather_tiles": (C.c_int, []),
-    "lgx_gather_batch_tiles": (C.c_int, []),
+    "lgx_gather_tile_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32)]),
     "lgx_gather_noalloc": (C.c_int, [C.POINTER(GatherParams), C.POINTER(GBuffer), C.POINTER(Pyramid),
                                      C.POINTER(Image), C.c_void_p]),
     "lgx_mip_build": (C.c_int, [C.POINTER(Image), C.c_int32, C.c_int32, C.c_void_p]),

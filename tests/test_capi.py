@@ -115,12 +115,9 @@ def test_batch_argument_checks():
     pyr.radiance[0] = lgx.image(fake, w, hh, lgx.RGBA16F)
     out = lgx.image(fake, w, hh, lgx.RGBA32F)
     ws = (C.c_void_p * 1)(fake)
-    # shapes from lgx_gather_batch_tiles() up re-lay the pyramid in the
-    # workspace per call: single-view only
-    nb = L.lgx_gather_batch_tiles()
-    assert 0 < nb < L.lgx_gather_tiles()
+    swz = swizzling_tiles()
     for field, value, code in (("precision", lgx.PREC_F64, lgx.EFORMAT), ("tile", 99, lgx.EUNSUPPORTED),
-                               ("tile", nb, lgx.EUNSUPPORTED), ("dirs", 4, lgx.EWORKSPACE)):
+                               ("dirs", 4, lgx.EWORKSPACE)):
         q = lgx.GatherParams()
         C.memmove(C.byref(q), C.byref(p), C.sizeof(q))
         setattr(q, field, value)
@@ -131,8 +128,37 @@ def test_batch_argument_checks():
     # the allocation-free gather keeps only tables in its array
     q = lgx.GatherParams()
     C.memmove(C.byref(q), C.byref(p), C.sizeof(q))
-    q.tile = nb
+    q.tile = swz[0]
     assert L.lgx_gather_noalloc(C.byref(q), C.byref(g), C.byref(pyr), C.byref(out), None) == lgx.EUNSUPPORTED
+
+
+def tile_info(t):
+    import ctypes as C
+    info = (C.c_int32 * 5)()
+    assert lgx.lib().lgx_gather_tile_info(t, info) == lgx.OK
+    return list(info)
+
+
+def swizzling_tiles():
+    return [t for t in range(lgx.lib().lgx_gather_tiles()) if tile_info(t)[4]]
+
+
+def test_gather_tile_info():
+    """Every launch shape is described; each swizzling shape has a twin with
+    the same tile, registers and step loop that reads the levels as built."""
+    import ctypes as C
+    L = lgx.lib()
+    n = L.lgx_gather_tiles()
+    infos = [tile_info(t) for t in range(n)]
+    assert infos[0] == [16, 16, 1, 0, 0]  # the default shape
+    for w, h, occ, pipe, swz in infos:
+        assert w * h == 256 and occ in (1, 8) and pipe in (0, 1) and swz in (0, 1)
+    plain = [tuple(i[:4]) for i in infos if not i[4]]
+    twins = [tuple(i[:4]) for i in infos if i[4]]
+    assert len(set(plain)) == len(plain) and sorted(plain) == sorted(twins) and len(twins) > 0
+    info = (C.c_int32 * 5)()
+    assert L.lgx_gather_tile_info(n, info) == lgx.EINVAL and L.lgx_gather_tile_info(-1, info) == lgx.EINVAL
+    assert L.lgx_gather_tile_info(0, None) == lgx.EINVAL
 
 
 def test_swizzled_workspace_size():
@@ -141,13 +167,13 @@ def test_swizzled_workspace_size():
     others, and samplers or precisions that never swizzle, do not."""
     import ctypes as C
     L = lgx.lib()
-    nb = L.lgx_gather_batch_tiles()
     p = lgx.GatherParams()
     p.width, p.height, p.dirs, p.max_steps, p.grid, p.footprint_dirs = 1920, 1080, 4, 0, 8, 256
     p.sampler = lgx.SAMPLER_HIZ_POINT
     base = L.lgx_gather_workspace_bytes(C.byref(p))
     texels = sum((1920 >> l) * (1080 >> l) for l in range(3))
-    for t in range(nb, L.lgx_gather_tiles()):
+    swz = swizzling_tiles()
+    for t in swz:
         p.tile = t
         extra = L.lgx_gather_workspace_bytes(C.byref(p)) - base
         assert 20 * texels <= extra <= 20 * texels + 7 * 256

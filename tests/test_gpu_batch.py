@@ -54,7 +54,7 @@ def test_gather_batch_matches_single_view(dev, views, footprint):
     gb = dev.GatherBatch(items)
     expect_blocks = sum(((w // 4 + 15) // 16) * ((h // 4 + 15) // 16) * 16 for (w, h) in SIZES)
     assert gb.blocks == expect_blocks
-    for tile in range(lgx.lib().lgx_gather_batch_tiles()):
+    for tile in range(lgx.lib().lgx_gather_tiles()):
         gb.set_tile(tile)
         refs = [single(dev, g, v, p, tile, footprint=footprint) for (g, v, p) in views]
         for wg in (0, 8, 64, 1000):
@@ -104,7 +104,7 @@ def test_batch_matches_pieces_and_orders_latencies(dev, views, net):
 def test_batch_capture_replay_and_autotune(dev, views, net):
     b = dev.Batch(views, dirs=4, max_steps=64, levels=6, net=net)
     rep = b.autotune()
-    assert len(rep["table"]) == 4 * lgx.lib().lgx_gather_batch_tiles()
+    assert len(rep["table"]) == 4 * lgx.lib().lgx_gather_tiles()
     assert b.shape() == (rep["best"]["tile"], rep["best"]["workgroups"])
     b.run()
     torch.cuda.synchronize()
@@ -122,6 +122,33 @@ def test_batch_capture_replay_and_autotune(dev, views, net):
         for i in range(len(views)):
             assert b.output(i, 1).tobytes() == eager[i].tobytes(), (r, i)
         assert all(x > 0 for x in b.latencies())
+    b.close()
+
+
+def test_batch_swizzled_shape_shared_workspace(dev, net):
+    """A batch created with a swizzling launch shape (8x8 pattern, F = 256,
+    no step cap: the denoised configuration) where two views share a
+    resolution, hence a workspace: the batch keeps one swizzled copy per view,
+    and every view equals the single-view gather with the plain layout."""
+    import ctypes as C
+    info = (C.c_int32 * 5)()
+    tile = next(t for t in range(lgx.lib().lgx_gather_tiles())
+                if lgx.lib().lgx_gather_tile_info(t, info) == 0 and info[4] and info[3])
+    sizes = [(256, 128), (192, 128), (256, 128)]
+    views = []
+    for i, (w, h) in enumerate(sizes):
+        sc = lgx.make_scene(60 + i, 8, 2)
+        view, proj = lgx.scene_matrices(sc, w, h)
+        views.append((dev.GBufferDev.alloc(w, h, "mixed").render(sc), view, proj))
+    b = dev.Batch(views, dirs=4, max_steps=0, levels=6, grid=8, footprint_dirs=256, tile=tile, net=net)
+    assert b.shape()[0] == tile
+    for rep in range(2):
+        b.run()
+        torch.cuda.synchronize()
+        for i, (g, view, proj) in enumerate(views):
+            pyr = dev.PyramidDev(g, 6, lgx.SAMPLER_HIZ_POINT).build()
+            ref = dev.Gather(g.w, g.h, 4, 0, lgx.SAMPLER_HIZ_POINT, grid=8, footprint_dirs=256)(view, proj, g, pyr)
+            assert b.output(i, 0).tobytes() == ref.numpy().tobytes(), (rep, i)
     b.close()
 
 

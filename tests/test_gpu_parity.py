@@ -242,11 +242,13 @@ def test_gather_tiles_identical(dev, c1, tile):
                                           (8, 256, 144, (40, 24, 130, 70)), (2, 97, 61, None)])
 @pytest.mark.parametrize("tile", [16, 20, 24, 28])
 def test_gather_swizzled_levels_identical(dev, grid, w, h, roi, tile):
-    """The launch shapes from lgx_gather_batch_tiles() up read column-swizzled
-    copies of the fine levels (RGBA16F radiance here, widths that are not
-    multiples of the pattern, a region of interest, two gathers in a row on
-    one workspace): the same bits as the plain layout."""
-    assert tile >= lgx.lib().lgx_gather_batch_tiles()
+    """Swizzling launch shapes read column-swizzled copies of the fine levels
+    (RGBA16F radiance here, widths that are not multiples of the pattern, a
+    region of interest, two gathers in a row on one workspace): the same bits
+    as the plain layout."""
+    import ctypes as C
+    info = (C.c_int32 * 5)()
+    assert lgx.lib().lgx_gather_tile_info(tile, info) == 0 and info[4] == 1
     sc, view, proj, gb = host_scene(11, w, h, "mixed", 8, 2)
     g = dev.GBufferDev.from_host(gb)
     pyr = dev.PyramidDev(g, 6, lgx.SAMPLER_HIZ_POINT).build()

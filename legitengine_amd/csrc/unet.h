@@ -73,6 +73,10 @@ struct WgradArgs {
 };
 
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
+// Inference: the first convolution from the G-buffer (a0: gin_* set, as for
+// conv3x3) and the second one with its 2x2 pool (a1: out0, pool_out) in one
+// kernel; the first layer's output stays in LDS.
+hipError_t conv_first_pair(const ConvArgs &a0, const ConvArgs &a1, hipStream_t s);
 int conv3x3_tiles(int cout);  // register tiles offered for this output channel count
 int conv3x3_tile_lds(int cout, int tile);  // 1 if that tile is an LDS tile covering all cout channels
 int conv3x3_tile_pool(int cout, int tile);  // 1 if that tile can write the 2x2 pooled output (pool_out)

@@ -302,8 +302,6 @@ class Net {
           "head_fwd");
   }
 
-  // inference straight from a G-buffer and the noisy image (same values as
-  // packing the records and calling forward)
   // An LDS tile of a 32-channel convolution: the autotuned one for layer l, else the first offered.
   int lds_tile32(int l) const {
     if (conv3x3_tile_lds(32, fwd_tile[l])) return fwd_tile[l];
@@ -312,10 +310,14 @@ class Net {
     return 0;
   }
 
+  // Inference straight from a G-buffer and the noisy image: the first two
+  // convolutions in one kernel (the first packs its input records from the
+  // G-buffer while staging them, its output stays in LDS, and it sums two
+  // taps per MFMA, so its values differ from forward's by the rounding of
+  // the partial sums), then layers 2-8, then the last convolution with the
+  // head in its epilogue.
   void denoise(const lgx_image &noisy, const lgx_gbuffer &g, const lgx_image &out_, hipStream_t s) {
     const int h = noisy.h, w = noisy.w;
-    // the first convolution packs its input records from the G-buffer and
-    // the noisy image while staging them (no 32-channel input tensor)
     {
       ConvArgs c{};
       c.ca = 32;
@@ -326,9 +328,7 @@ class Net {
       c.bias = params + b_off[0];
       c.cout = 32;
       c.relu = 1;
-      c.out0 = e1a;
       c.split = 32;
-      c.tile = lds_tile32(0);
       c.gin_fmt = g.normal.format == LGX_RG16F ? 1 : 2;
       c.gin_noisy = (const float4 *)noisy.ptr;
       c.gin_noisy_pitch = noisy.pitch;
@@ -338,9 +338,21 @@ class Net {
       c.gin_normal_pitch = g.normal.pitch;
       c.gin_albedo = g.albedo.ptr;
       c.gin_albedo_pitch = g.albedo.pitch;
-      check(conv3x3(c, s), "conv3x3 from the G-buffer");
+      ConvArgs c1{};
+      c1.ca = 32;
+      c1.n = 1;
+      c1.h = h;
+      c1.w = w;
+      c1.wt = wf[1];
+      c1.bias = params + b_off[1];
+      c1.cout = 32;
+      c1.relu = 1;
+      c1.out0 = e1;
+      c1.split = 32;
+      c1.pool_out = p1;
+      check(conv_first_pair(c, c1, s), "conv_first_pair");
     }
-    body(1, h, w, s, 9, 1);
+    body(1, h, w, s, 9, 2);
     // the last convolution with the 1x1 head fused into its epilogue (an LDS
     // tile over all 32 channels: the autotuned one, else the first offered)
     ConvArgs c{};

@@ -644,6 +644,245 @@ __global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_
   }
 }
 
+// Inference: the first two convolutions in one kernel.  A workgroup's tile
+// is 8 x 32 pixels of the second convolution's output.  Its input, the
+// first convolution's output over the tile and a 1-pixel halo (10 x 34), is
+// computed into LDS from the input records of the tile and a 2-pixel halo
+// (12 x 36, packed from the G-buffer while staging) and never goes to HBM:
+// about half of the two layers' HBM traffic, for 1.33x the first layer's
+// matrix work (the halo).  The first layer's records have 9 channels,
+// padded to 16, so one tap per 32-deep MFMA would leave half of K zero;
+// here taps 2p and 2p + 1 share an MFMA (k = 16 * (tap & 1) + channel; the
+// tenth tap is zero): 5 MFMAs per output block instead of 9.  (The training
+// forward runs the first layer one tap per MFMA, so its values differ from
+// these by the rounding of the partial sums.)  The second convolution is
+// conv3x3_lds_kernel<2, 2, 2> on that LDS tile: the same operand order and
+// the same epilogue, with the 2x2 pool.  Persistent: a workgroup walks the
+// tiles t, t + gridDim.x, ...; the next tile's raw texels are in flight
+// during the current tile's matrix work.
+template <int GIN>
+__global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs a0, const ConvArgs a1, int tiles_x,
+                                                                 int tiles_y) {
+  constexpr int MB = 2, WR = 2, WP = 2;
+  constexpr int TH = 4 * WR, TW = 16 * WP;  // output tile
+  constexpr int IH = TH + 2, IW = TW + 2;   // first-layer outputs (the second layer's input tile)
+  constexpr int AH = TH + 4, AW = TW + 4;   // input records
+  constexpr int NA = AH * AW, NB = IH * IW;
+  constexpr int PLANE_A = NA * 8, PLANE_B = (NB * 8 + 127) / 128 * 128, PLANE_W = 9 * MB * 16 * 8;
+  constexpr int A_BLOCKS = (NB + 15) / 16;   // 16-pixel blocks of first-layer outputs
+  constexpr int A_PER = (A_BLOCKS + 3) / 4;  // per wave (blocks wave, wave + 4, ...)
+  constexpr int A_GROUP = 3;                 // blocks per pass (shared weight registers)
+  constexpr int GIN_PER = (NA + 255) / 256;
+  constexpr int W_PIECES = 9 * MB * 16 * 4, W_PER = (W_PIECES + 255) / 256;
+  static_assert(A_PER % A_GROUP == 0, "block groups");
+  __shared__ __attribute__((aligned(16))) bf16 s_a[2 * PLANE_A];  // records: channels 0-7, 8-15
+  __shared__ __attribute__((aligned(16))) bf16 s_b[4 * PLANE_B];  // first-layer outputs, 4 x 8 channels
+  __shared__ __attribute__((aligned(16))) bf16 s_w[4 * PLANE_W];  // second-layer weights (resident)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 15, kg = lane >> 4;
+  const int T = tiles_x * tiles_y;
+  int t = blockIdx.x;
+  if (t >= T) return;  // the whole workgroup, before any barrier
+  const bf16x8 zero8 = {};
+  // second-layer weights -> LDS (conv3x3_lds_kernel's layout and pieces)
+  {
+    bf16x8 pw[W_PER];
+#pragma unroll
+    for (int i = 0; i < W_PER; i++) {
+      const int q = min(tid + i * 256, W_PIECES - 1);
+      const int row = ((q >> 6) << 4) + (q & 15), part = (q >> 4) & 3;
+      const int tap = row / (MB * 16), co = row % (MB * 16);
+      pw[i] = *(const bf16x8 *)(a1.wt + (size_t)tap * 32 * 32 + (co * 4 + part) * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < W_PER; i++) {
+      const int q = tid + i * 256;
+      const int row = ((q >> 6) << 4) + (q & 15), part = (q >> 4) & 3;
+      if (q < W_PIECES) *(bf16x8 *)(s_w + part * PLANE_W + row * 8) = pw[i];
+    }
+  }
+  // first-layer weights in registers: the A operand of pair p, block mb
+  // (output channel mb * 16 + col; k group kg = tap 2p + (kg >> 1),
+  // channels 8 * (kg & 1) .. + 7 of the packed [tap][cout][32] weights)
+  bf16x8 w0[5][MB];
+  int toff[5];  // this lane's tap offset (elements) in the record tile, per pair
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    const int tap = 2 * p + (kg >> 1);
+    toff[p] = tap < 9 ? ((tap / 3) * AW + tap % 3) * 8 : 0;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+      w0[p][mb] = tap < 9 ? *(const bf16x8 *)(a0.wt + ((size_t)tap * 32 + mb * 16 + col) * 32 + (kg & 1) * 8) : zero8;
+  }
+  float bias0[MB][4], bias1[MB][4];
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      bias0[mb][i] = a0.bias[mb * 16 + kg * 4 + i];
+      bias1[mb][i] = a1.bias[mb * 16 + kg * 4 + i];
+    }
+  GinRaw<GIN> graw[GIN_PER];
+  uint32_t valid = 0;
+  auto fetch = [&](int tt) {
+    const int x0 = (tt % tiles_x) * TW, y0 = (tt / tiles_x) * TH;
+    valid = 0;
+#pragma unroll
+    for (int i = 0; i < GIN_PER; i++) {
+      const int p = min(tid + i * 256, NA - 1);
+      const int y = y0 - 2 + p / AW, x = x0 - 2 + p % AW;
+      const bool ok = y >= 0 && y < a0.h && x >= 0 && x < a0.w;
+      graw[i] = gin_load<GIN>(a0, min(max(y, 0), a0.h - 1), min(max(x, 0), a0.w - 1));
+      valid |= (uint32_t)ok << i;
+    }
+  };
+  fetch(t);
+  for (;;) {
+    const int x0 = (t % tiles_x) * TW, y0 = (t / tiles_x) * TH;
+    // records -> LDS (zero outside the image: the first layer's padding)
+#pragma unroll
+    for (int i = 0; i < GIN_PER; i++) {
+      const int p = tid + i * 256;
+      bf16x8 v0 = {}, v1 = {};
+      if ((valid >> i) & 1u) gin_pack<GIN>(graw[i], v0, v1);
+      if (p < NA) {
+        *(bf16x8 *)(s_a + p * 8) = v0;
+        *(bf16x8 *)(s_a + PLANE_A + p * 8) = v1;
+      }
+    }
+    __syncthreads();
+    if (t + (int)gridDim.x < T) fetch(t + (int)gridDim.x);  // in flight during the matrix work
+    // first layer: this wave's blocks, A_GROUP at a time (not unrolled: one
+    // group's operands live at a time)
+#pragma unroll 1
+    for (int g = 0; g < A_PER; g += A_GROUP) {
+      // this lane's first-layer output pixel in each block of the group
+      // (clamped: the tail block's extra lanes compute a duplicate that is
+      // not stored)
+      int abase[A_GROUP];
+#pragma unroll
+      for (int j = 0; j < A_GROUP; j++) {
+        const int q = min((wave + 4 * (g + j)) * 16 + col, NB - 1);
+        abase[j] = ((q / IW) * AW + q % IW) * 8 + (kg & 1) * PLANE_A;
+      }
+      f32x4 acc0[A_GROUP][MB];
+#pragma unroll
+      for (int j = 0; j < A_GROUP; j++)
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) acc0[j][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int p = 0; p < 5; p++)
+#pragma unroll
+        for (int j = 0; j < A_GROUP; j++) {
+          if (wave + 4 * (g + j) >= A_BLOCKS) continue;  // uniform
+          bf16x8 bx = *(const bf16x8 *)(s_a + abase[j] + toff[p]);
+          if (p == 4 && kg >= 2) bx = zero8;  // the tenth tap: zero weights, and no 0 * inf
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++)
+            acc0[j][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0[p][mb], bx, acc0[j][mb], 0, 0, 0);
+        }
+      // bias, ReLU, BF16 (conv_store_row's rounding), zero outside the image
+      // (the second layer's padding) -> LDS
+#pragma unroll
+      for (int j = 0; j < A_GROUP; j++) {
+        const int blk = wave + 4 * (g + j);
+        const int q = blk * 16 + col;
+        if (blk >= A_BLOCKS || q >= NB) continue;
+        const int y = y0 - 1 + q / IW, x = x0 - 1 + q % IW;
+        const bool in = y >= 0 && y < a0.h && x >= 0 && x < a0.w;
+        typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+          bf16x4 o;
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            float v = acc0[j][mb][i] + bias0[mb][i];
+            v = v > 0.f ? v : 0.f;
+            o[i] = f2bf(in ? v : 0.f);
+          }
+          *(bf16x4 *)(s_b + (mb * 2 + (kg >> 1)) * PLANE_B + q * 8 + (kg & 1) * 4) = o;
+        }
+      }
+    }
+    __syncthreads();
+    // second layer on the LDS tile (conv3x3_lds_kernel's operand order)
+    f32x4 acc[WR][WP][MB];
+#pragma unroll
+    for (int r = 0; r < WR; r++)
+#pragma unroll
+      for (int pb = 0; pb < WP; pb++)
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) acc[r][pb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++) {
+      bf16x8 bx[WR + 2][WP];
+#pragma unroll
+      for (int rr = 0; rr < WR + 2; rr++)
+#pragma unroll
+        for (int pb = 0; pb < WP; pb++)
+          bx[rr][pb] = *(const bf16x8 *)(s_b + kg * PLANE_B + ((wave * WR + rr) * IW + pb * 16 + col + dx) * 8);
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        const int tap = dy * 3 + dx;
+        bf16x8 aw[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) aw[mb] = *(const bf16x8 *)(s_w + kg * PLANE_W + ((tap * MB + mb) * 16 + col) * 8);
+#pragma unroll
+        for (int r = 0; r < WR; r++)
+#pragma unroll
+          for (int pb = 0; pb < WP; pb++)
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+              acc[r][pb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[mb], bx[r + dy][pb], acc[r][pb][mb], 0, 0, 0);
+      }
+    }
+    // 2x2 pool of the stored (BF16) outputs (conv3x3_lds_kernel's pool
+    // epilogue: rows r, r + 1 of this lane, columns x, x + 1 of lanes col,
+    // col + 1, summed in avgpool_kernel's order)
+    {
+      const int hh = a1.h >> 1, ww = a1.w >> 1;
+#pragma unroll
+      for (int r = 0; r + 1 < WR; r += 2) {
+        const int y = y0 + wave * WR + r;
+#pragma unroll
+        for (int pb = 0; pb < WP; pb++) {
+          const int x = x0 + pb * 16 + col;
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) {
+            const int co = mb * 16 + kg * 4;
+            float tv[4], bv[4], tn[4], bn[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              float v0 = acc[r][pb][mb][i] + bias1[mb][i], v1 = acc[r + 1][pb][mb][i] + bias1[mb][i];
+              v0 = v0 > 0.f ? v0 : 0.f;
+              v1 = v1 > 0.f ? v1 : 0.f;
+              tv[i] = bf2f(f2bf(v0));
+              bv[i] = bf2f(f2bf(v1));
+              tn[i] = lane_xor1(tv[i]);
+              bn[i] = lane_xor1(bv[i]);
+            }
+            if (!(col & 1) && (y >> 1) < hh && (x >> 1) < ww) {
+              typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+              bf16x4 o;
+#pragma unroll
+              for (int i = 0; i < 4; i++) o[i] = f2bf((((tv[i] + tn[i]) + bv[i]) + bn[i]) * 0.25f);
+              *(bf16x4 *)(a1.pool_out + ((size_t)(y >> 1) * ww + (x >> 1)) * 32 + co) = o;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < WR; r++) {
+      const int y = y0 + wave * WR + r;
+      if (y >= a1.h) break;
+      conv_store_row<WP, MB>(a1, acc[r], bias1, 0, y, x0, col, kg, 0);
+    }
+    t += gridDim.x;
+    if (t >= T) break;
+  }
+}
+
 // Weight and bias gradient of a 3x3 convolution:
 //   dW[tap][ci][co] = sum_p X[p + shift(tap)][ci] * dY[p][co],  db[co] = sum_p dY[p][co]
 // One workgroup: one image, one 32-channel group of the input, a 64-pixel
@@ -1344,6 +1583,20 @@ hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
     case 12: return conv_mb<12>(a, t, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t conv_first_pair(const ConvArgs &a0, const ConvArgs &a1, hipStream_t s) {
+  if (a0.gin_fmt < 1 || a0.gin_fmt > 2 || a0.n != 1 || a0.cout != 32 || !a0.wt || !a0.bias || a1.n != 1 ||
+      a1.cout != 32 || a1.split != 32 || !a1.out0 || !a1.pool_out || !a1.wt || !a1.bias || a1.h != a0.h ||
+      a1.w != a0.w || a0.h < 1 || a0.w < 1)
+    return hipErrorInvalidValue;
+  const int tx = (a0.w + 31) / 32, ty = (a0.h + 7) / 8;
+  const int grid = std::min(tx * ty, 2 * cu_count());  // two workgroups per CU fit its LDS and registers
+  if (a0.gin_fmt == 1)
+    hipLaunchKernelGGL(conv_first_pair_kernel<1>, dim3((unsigned)grid), dim3(256), 0, s, a0, a1, tx, ty);
+  else
+    hipLaunchKernelGGL(conv_first_pair_kernel<2>, dim3((unsigned)grid), dim3(256), 0, s, a0, a1, tx, ty);
+  return hipGetLastError();
 }
 
 // Weight-gradient workgroups per launch by output-channel count: each stores

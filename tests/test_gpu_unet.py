@@ -148,7 +148,15 @@ def test_denoise_from_gbuffer_matches_records(dev):
     rec[0, ..., 6:9] = gb["albedo"][..., :3].astype(np.float32) * np.float32(1.0 / 255.0)
     ref = net.forward(torch.from_numpy(rec).cuda())
     torch.cuda.synchronize()
-    assert np.allclose(out.numpy(), ref.cpu().numpy()[0], atol=1e-6)
+    o, r = out.numpy(), ref.cpu().numpy()[0]
+    # the inference kernel sums two first-layer taps per MFMA (forward: one),
+    # so the two differ by the rounding of those partial sums, carried
+    # through the BF16 activations
+    assert np.isfinite(o).all()
+    assert rel(o[..., :3] - noisy[..., :3], r[..., :3] - noisy[..., :3]) <= 1e-2, \
+        rel(o[..., :3] - noisy[..., :3], r[..., :3] - noisy[..., :3])
+    again = net.denoise(g, dn).numpy()
+    assert again.tobytes() == o.tobytes()
 
 
 def test_register_tiles_bit_identical_and_autotune(dev):
@@ -193,12 +201,12 @@ def test_register_tiles_bit_identical_and_autotune(dev):
 
 
 def test_fused_head_every_tile(dev):
-    """Inference fuses both ends of the network into LDS convolution tiles:
-    the first convolution packs its input records from the G-buffer while
-    staging them, and the head runs in the epilogue of the last convolution
-    (each the autotuned tile, or the first LDS tile when the chosen tile has
-    no LDS form). Every tile of either layer gives the unfused forward's
-    output."""
+    """Inference fuses both ends of the network: the first two convolutions
+    run in one kernel from the G-buffer (whatever tiles layers 0 and 1 have),
+    and the head runs in the epilogue of the last convolution (the autotuned
+    tile, or the first LDS tile when the chosen tile has no LDS form). Every
+    tile gives the same bits, close to the unfused forward's output (the
+    first layer's partial sums are rounded differently)."""
     from legitengine_amd import lgx
     sc = lgx.make_scene(6, 8, 2)
     gb = lgx.render_gbuffer_cpu(sc, 80, 48, "mixed")
@@ -215,13 +223,15 @@ def test_fused_head_every_tile(dev):
     f0, d0 = net.tiles()
     n9 = lgx.lib().lgx_unet_conv_tiles(32)
     assert n9 >= 2
-    for layer in (0, 9):
+    first = net.denoise(g, dn).numpy()
+    assert rel(first[..., :3] - noisy[..., :3], ref[..., :3] - noisy[..., :3]) <= 1e-2
+    for layer in (0, 1, 9):
         for t in range(n9):
             f = list(f0)
             f[layer] = t
             net.set_tiles(f, d0)
             out = net.denoise(g, dn).numpy()
-            assert np.allclose(out, ref, rtol=0, atol=1e-6), (layer, t, np.abs(out - ref).max())
+            assert out.tobytes() == first.tobytes(), (layer, t, np.abs(out - first).max())
 
 
 def test_denoise_rejects_bad_images_on_the_host(dev):

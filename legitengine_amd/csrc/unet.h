@@ -36,11 +36,12 @@ struct ConvArgs {
   // the tile is staged, channels (noisy rgb, depth, normal xy, albedo rgb,
   // zeros), the values pack_input gives for those records; a and b are
   // unused.
-  int gin_fmt;  // 0: off; 1: RG16F normal + RGBA8 albedo; 2: RG32F + RGBA32F
+  int gin_fmt;  // 0: off; 1: RG16F normal + RGBA8 albedo; 2: RG32F + RGBA32F; 3: records (conv_first_pair)
   const float4 *gin_noisy;
   const float *gin_depth;
   const void *gin_normal, *gin_albedo;
   int gin_noisy_pitch, gin_depth_pitch, gin_normal_pitch, gin_albedo_pitch;
+  const float *gin_x12;  // conv_first_pair with gin_fmt 3: the input records [n][h][w][12]
   // Also write the 2x2 average pool of the output (NHWC, h/2 x w/2, cout
   // channels), summed as avgpool_kernel sums it; null = off.  Needs an LDS
   // tile whose waves hold an even number of rows (conv3x3_tile_pool).
@@ -73,9 +74,11 @@ struct WgradArgs {
 };
 
 hipError_t conv3x3(const ConvArgs &a, hipStream_t s);
-// Inference: the first convolution from the G-buffer (a0: gin_* set, as for
-// conv3x3) and the second one with its 2x2 pool (a1: out0, pool_out) in one
-// kernel; the first layer's output stays in LDS.
+// The first convolution from the G-buffer (a0: gin_* set, as for conv3x3,
+// n = 1) or from input records (gin_fmt 3, gin_x12, any n) and the second
+// one with its 2x2 pool (a1: out0, pool_out) in one kernel; the first
+// layer's output stays in LDS and also goes to a0.out0 when set (the
+// training forward keeps it for the backward pass).
 hipError_t conv_first_pair(const ConvArgs &a0, const ConvArgs &a1, hipStream_t s);
 int conv3x3_tiles(int cout);  // register tiles offered for this output channel count
 int conv3x3_tile_lds(int cout, int tile);  // 1 if that tile is an LDS tile covering all cout channels

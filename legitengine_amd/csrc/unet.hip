@@ -293,10 +293,43 @@ class Net {
     for (int l = from; l < upto; l++) fwd_layer(l, n, h, w, s);
   }
 
+  // the first two convolutions in one kernel (conv_first_pair): first-layer
+  // inputs from the records (x12) or, for denoise, the G-buffer; its output
+  // to e1a when kept (the training forward: the backward pass reads it)
+  void first_pair(ConvArgs &c0, int n, int h, int w, bf16 *keep, hipStream_t s) {
+    c0.ca = 32;
+    c0.n = n;
+    c0.h = h;
+    c0.w = w;
+    c0.wt = wf[0];
+    c0.bias = params + b_off[0];
+    c0.cout = 32;
+    c0.relu = 1;
+    c0.split = 32;
+    c0.out0 = keep;
+    ConvArgs c1{};
+    c1.ca = 32;
+    c1.n = n;
+    c1.h = h;
+    c1.w = w;
+    c1.wt = wf[1];
+    c1.bias = params + b_off[1];
+    c1.cout = 32;
+    c1.relu = 1;
+    c1.out0 = e1;
+    c1.split = 32;
+    c1.pool_out = p1;
+    check(conv_first_pair(c0, c1, s), "conv_first_pair");
+  }
+
   void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
     const size_t P = (size_t)n * h * w;
-    check(pack_input(x12, P, x32, s), "pack_input");
-    body(n, h, w, s);
+    check(pack_input(x12, P, x32, s), "pack_input");  // the first layer's weight gradient reads x32
+    ConvArgs c0{};
+    c0.gin_fmt = 3;
+    c0.gin_x12 = x12;
+    first_pair(c0, n, h, w, e1a, s);
+    body(n, h, w, s, 10, 2);
     check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, w, out_pitch > 0 ? out_pitch : w,
                    (float4 *)out_, s),
           "head_fwd");
@@ -310,25 +343,14 @@ class Net {
     return 0;
   }
 
-  // Inference straight from a G-buffer and the noisy image: the first two
-  // convolutions in one kernel (the first packs its input records from the
-  // G-buffer while staging them, its output stays in LDS, and it sums two
-  // taps per MFMA, so its values differ from forward's by the rounding of
-  // the partial sums), then layers 2-8, then the last convolution with the
-  // head in its epilogue.
+  // Inference straight from a G-buffer and the noisy image (the values of
+  // forward on the records they give): the first two convolutions in one
+  // kernel that packs the records from the G-buffer while staging them,
+  // then layers 2-8, then the last convolution with the head in its epilogue.
   void denoise(const lgx_image &noisy, const lgx_gbuffer &g, const lgx_image &out_, hipStream_t s) {
     const int h = noisy.h, w = noisy.w;
     {
       ConvArgs c{};
-      c.ca = 32;
-      c.n = 1;
-      c.h = h;
-      c.w = w;
-      c.wt = wf[0];
-      c.bias = params + b_off[0];
-      c.cout = 32;
-      c.relu = 1;
-      c.split = 32;
       c.gin_fmt = g.normal.format == LGX_RG16F ? 1 : 2;
       c.gin_noisy = (const float4 *)noisy.ptr;
       c.gin_noisy_pitch = noisy.pitch;
@@ -338,19 +360,7 @@ class Net {
       c.gin_normal_pitch = g.normal.pitch;
       c.gin_albedo = g.albedo.ptr;
       c.gin_albedo_pitch = g.albedo.pitch;
-      ConvArgs c1{};
-      c1.ca = 32;
-      c1.n = 1;
-      c1.h = h;
-      c1.w = w;
-      c1.wt = wf[1];
-      c1.bias = params + b_off[1];
-      c1.cout = 32;
-      c1.relu = 1;
-      c1.out0 = e1;
-      c1.split = 32;
-      c1.pool_out = p1;
-      check(conv_first_pair(c, c1, s), "conv_first_pair");
+      first_pair(c, 1, h, w, nullptr, s);
     }
     body(1, h, w, s, 9, 2);
     // the last convolution with the 1x1 head fused into its epilogue (an LDS

@@ -247,8 +247,20 @@ template <> struct GinRaw<2> {
   float2 n;
   float4 al;
 };
-template <int GIN> LGX_D GinRaw<GIN> gin_load(const ConvArgs &a, int y, int x) {
+template <> struct GinRaw<3> {  // the first 9 floats of an input record
+  float4 a, b;
+  float c;
+};
+// (GIN = 3 reads image n's records; the G-buffer formats have one image)
+template <int GIN> LGX_D GinRaw<GIN> gin_load(const ConvArgs &a, int y, int x, int n = 0) {
   GinRaw<GIN> r;
+  if constexpr (GIN == 3) {
+    const float *rec = a.gin_x12 + (((size_t)n * a.h + y) * a.w + x) * 12;
+    r.a = *(const float4 *)rec;
+    r.b = *(const float4 *)(rec + 4);
+    r.c = rec[8];
+    return r;
+  } else {
   r.c = a.gin_noisy[(size_t)y * a.gin_noisy_pitch + x];
   r.d = a.gin_depth[(size_t)y * a.gin_depth_pitch + x];
   if constexpr (GIN == 1) {
@@ -259,10 +271,17 @@ template <int GIN> LGX_D GinRaw<GIN> gin_load(const ConvArgs &a, int y, int x) {
     r.al = ((const float4 *)a.gin_albedo)[(size_t)y * a.gin_albedo_pitch + x];
   }
   return r;
+  }
 }
 // The first 16 channels of the pixel's input record (pack_input_kernel's
 // values for the record the G-buffer and the noisy image give).
 template <int GIN> LGX_D void gin_pack(const GinRaw<GIN> &r, bf16x8 &v0, bf16x8 &v1) {
+  if constexpr (GIN == 3) {
+    v0 = bf16x8{f2bf(r.a.x), f2bf(r.a.y), f2bf(r.a.z), f2bf(r.a.w), f2bf(r.b.x), f2bf(r.b.y), f2bf(r.b.z), f2bf(r.b.w)};
+    v1 = bf16x8{};
+    v1[0] = f2bf(r.c);
+    return;
+  } else {
   float nx, ny, ar, ag, ab;
   if constexpr (GIN == 1) {
     nx = h2f((uint16_t)(r.n & 0xffffu));
@@ -281,6 +300,7 @@ template <int GIN> LGX_D void gin_pack(const GinRaw<GIN> &r, bf16x8 &v0, bf16x8 
   v0 = bf16x8{f2bf(r.c.x), f2bf(r.c.y), f2bf(r.c.z), f2bf(r.d), f2bf(nx), f2bf(ny), f2bf(ar), f2bf(ag)};
   v1 = bf16x8{};
   v1[0] = f2bf(ab);
+  }
 }
 
 // The value of the neighbouring lane (lane ^ 1): a DPP quad permutation
@@ -644,18 +664,19 @@ __global__ __launch_bounds__(256, conv_lds_blocks(MB, WR, WP)) void conv3x3_lds_
   }
 }
 
-// Inference: the first two convolutions in one kernel.  A workgroup's tile
-// is 8 x 32 pixels of the second convolution's output.  Its input, the
-// first convolution's output over the tile and a 1-pixel halo (10 x 34), is
-// computed into LDS from the input records of the tile and a 2-pixel halo
-// (12 x 36, packed from the G-buffer while staging) and never goes to HBM:
-// about half of the two layers' HBM traffic, for 1.33x the first layer's
-// matrix work (the halo).  The first layer's records have 9 channels,
-// padded to 16, so one tap per 32-deep MFMA would leave half of K zero;
-// here taps 2p and 2p + 1 share an MFMA (k = 16 * (tap & 1) + channel; the
-// tenth tap is zero): 5 MFMAs per output block instead of 9.  (The training
-// forward runs the first layer one tap per MFMA, so its values differ from
-// these by the rounding of the partial sums.)  The second convolution is
+// The first two convolutions in one kernel (inference and the training
+// forward).  A workgroup's tile is 8 x 32 pixels of the second convolution's
+// output.  Its input, the first convolution's output over the tile and a
+// 1-pixel halo (10 x 34), is computed into LDS from the input records of the
+// tile and a 2-pixel halo (12 x 36, packed from the G-buffer or the FP32
+// records while staging, as pack_input_kernel packs them) and goes to HBM
+// only when the backward pass needs it: about half of the two layers' HBM
+// traffic, for 1.33x the first layer's matrix work (the halo).  The first
+// layer's records have 9 channels, padded to 16, so one tap per 32-deep MFMA
+// would leave half of K zero; here taps 2p and 2p + 1 share an MFMA
+// (k = 16 * (tap & 1) + channel; the tenth tap is zero): 5 MFMAs per output
+// block instead of 9.  (conv3x3 on the packed records, one tap per MFMA,
+// rounds the partial sums differently.)  The second convolution is
 // conv3x3_lds_kernel<2, 2, 2> on that LDS tile: the same operand order and
 // the same epilogue, with the 2x2 pool.  Persistent: a workgroup walks the
 // tiles t, t + gridDim.x, ...; the next tile's raw texels are in flight
@@ -680,7 +701,7 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
   __shared__ __attribute__((aligned(16))) bf16 s_w[4 * PLANE_W];  // second-layer weights (resident)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int col = lane & 15, kg = lane >> 4;
-  const int T = tiles_x * tiles_y;
+  const int T = tiles_x * tiles_y * a0.n;  // image-major
   int t = blockIdx.x;
   if (t >= T) return;  // the whole workgroup, before any barrier
   const bf16x8 zero8 = {};
@@ -725,20 +746,22 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
   GinRaw<GIN> graw[GIN_PER];
   uint32_t valid = 0;
   auto fetch = [&](int tt) {
-    const int x0 = (tt % tiles_x) * TW, y0 = (tt / tiles_x) * TH;
+    const int u = tt % (tiles_x * tiles_y), n = tt / (tiles_x * tiles_y);
+    const int x0 = (u % tiles_x) * TW, y0 = (u / tiles_x) * TH;
     valid = 0;
 #pragma unroll
     for (int i = 0; i < GIN_PER; i++) {
       const int p = min(tid + i * 256, NA - 1);
       const int y = y0 - 2 + p / AW, x = x0 - 2 + p % AW;
       const bool ok = y >= 0 && y < a0.h && x >= 0 && x < a0.w;
-      graw[i] = gin_load<GIN>(a0, min(max(y, 0), a0.h - 1), min(max(x, 0), a0.w - 1));
+      graw[i] = gin_load<GIN>(a0, min(max(y, 0), a0.h - 1), min(max(x, 0), a0.w - 1), n);
       valid |= (uint32_t)ok << i;
     }
   };
   fetch(t);
   for (;;) {
-    const int x0 = (t % tiles_x) * TW, y0 = (t / tiles_x) * TH;
+    const int u = t % (tiles_x * tiles_y), n = t / (tiles_x * tiles_y);
+    const int x0 = (u % tiles_x) * TW, y0 = (u / tiles_x) * TH;
     // records -> LDS (zero outside the image: the first layer's padding)
 #pragma unroll
     for (int i = 0; i < GIN_PER; i++) {
@@ -788,8 +811,11 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
         const int blk = wave + 4 * (g + j);
         const int q = blk * 16 + col;
         if (blk >= A_BLOCKS || q >= NB) continue;
-        const int y = y0 - 1 + q / IW, x = x0 - 1 + q % IW;
+        const int qy = q / IW, qx = q % IW;
+        const int y = y0 - 1 + qy, x = x0 - 1 + qx;
         const bool in = y >= 0 && y < a0.h && x >= 0 && x < a0.w;
+        // the tile's own pixels (not the halo) also to HBM, when asked
+        const bool keep = a0.out0 && in && qy >= 1 && qy <= TH && qx >= 1 && qx <= TW;
         typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
         for (int mb = 0; mb < MB; mb++) {
@@ -801,6 +827,7 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
             o[i] = f2bf(in ? v : 0.f);
           }
           *(bf16x4 *)(s_b + (mb * 2 + (kg >> 1)) * PLANE_B + q * 8 + (kg & 1) * 4) = o;
+          if (keep) *(bf16x4 *)(a0.out0 + (((size_t)n * a0.h + y) * a0.w + x) * 32 + mb * 16 + kg * 4) = o;
         }
       }
     }
@@ -866,7 +893,7 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
               bf16x4 o;
 #pragma unroll
               for (int i = 0; i < 4; i++) o[i] = f2bf((((tv[i] + tn[i]) + bv[i]) + bn[i]) * 0.25f);
-              *(bf16x4 *)(a1.pool_out + ((size_t)(y >> 1) * ww + (x >> 1)) * 32 + co) = o;
+              *(bf16x4 *)(a1.pool_out + (((size_t)n * hh + (y >> 1)) * ww + (x >> 1)) * 32 + co) = o;
             }
           }
         }
@@ -876,7 +903,7 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
     for (int r = 0; r < WR; r++) {
       const int y = y0 + wave * WR + r;
       if (y >= a1.h) break;
-      conv_store_row<WP, MB>(a1, acc[r], bias1, 0, y, x0, col, kg, 0);
+      conv_store_row<WP, MB>(a1, acc[r], bias1, n, y, x0, col, kg, 0);
     }
     t += gridDim.x;
     if (t >= T) break;
@@ -1586,16 +1613,19 @@ hipError_t conv3x3(const ConvArgs &a, hipStream_t s) {
 }
 
 hipError_t conv_first_pair(const ConvArgs &a0, const ConvArgs &a1, hipStream_t s) {
-  if (a0.gin_fmt < 1 || a0.gin_fmt > 2 || a0.n != 1 || a0.cout != 32 || !a0.wt || !a0.bias || a1.n != 1 ||
-      a1.cout != 32 || a1.split != 32 || !a1.out0 || !a1.pool_out || !a1.wt || !a1.bias || a1.h != a0.h ||
-      a1.w != a0.w || a0.h < 1 || a0.w < 1)
+  if (a0.gin_fmt < 1 || a0.gin_fmt > 3 || (a0.gin_fmt == 3 ? !a0.gin_x12 : a0.n != 1) || a0.n < 1 ||
+      a0.cout != 32 || !a0.wt || !a0.bias || a1.n != a0.n || a1.cout != 32 || a1.split != 32 || !a1.out0 ||
+      !a1.pool_out || !a1.wt || !a1.bias || a1.h != a0.h || a1.w != a0.w || a0.h < 1 || a0.w < 1)
     return hipErrorInvalidValue;
   const int tx = (a0.w + 31) / 32, ty = (a0.h + 7) / 8;
-  const int grid = std::min(tx * ty, 2 * cu_count());  // two workgroups per CU fit its LDS and registers
+  const long tiles = (long)tx * ty * a0.n;
+  const int grid = (int)std::min(tiles, 2L * cu_count());  // two workgroups per CU fit its LDS and registers
   if (a0.gin_fmt == 1)
     hipLaunchKernelGGL(conv_first_pair_kernel<1>, dim3((unsigned)grid), dim3(256), 0, s, a0, a1, tx, ty);
-  else
+  else if (a0.gin_fmt == 2)
     hipLaunchKernelGGL(conv_first_pair_kernel<2>, dim3((unsigned)grid), dim3(256), 0, s, a0, a1, tx, ty);
+  else
+    hipLaunchKernelGGL(conv_first_pair_kernel<3>, dim3((unsigned)grid), dim3(256), 0, s, a0, a1, tx, ty);
   return hipGetLastError();
 }
 

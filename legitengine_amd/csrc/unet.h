@@ -42,6 +42,7 @@ struct ConvArgs {
   const void *gin_normal, *gin_albedo;
   int gin_noisy_pitch, gin_depth_pitch, gin_normal_pitch, gin_albedo_pitch;
   const float *gin_x12;  // conv_first_pair with gin_fmt 3: the input records [n][h][w][12]
+  bf16 *rec_out;         // conv_first_pair: optional, the packed 32-channel records of every pixel
   // Also write the 2x2 average pool of the output (NHWC, h/2 x w/2, cout
   // channels), summed as avgpool_kernel sums it; null = off.  Needs an LDS
   // tile whose waves hold an even number of rows (conv3x3_tile_pool).

@@ -324,10 +324,10 @@ class Net {
 
   void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
     const size_t P = (size_t)n * h * w;
-    check(pack_input(x12, P, x32, s), "pack_input");  // the first layer's weight gradient reads x32
     ConvArgs c0{};
     c0.gin_fmt = 3;
     c0.gin_x12 = x12;
+    c0.rec_out = x32;  // the packed records: the first layer's weight gradient reads them
     first_pair(c0, n, h, w, e1a, s);
     body(n, h, w, s, 10, 2);
     check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, w, out_pitch > 0 ? out_pitch : w,

@@ -762,7 +762,9 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
   for (;;) {
     const int u = t % (tiles_x * tiles_y), n = t / (tiles_x * tiles_y);
     const int x0 = (u % tiles_x) * TW, y0 = (u / tiles_x) * TH;
-    // records -> LDS (zero outside the image: the first layer's padding)
+    // records -> LDS (zero outside the image: the first layer's padding);
+    // the tile's own pixels also to rec_out when asked (pack_input_kernel's
+    // layout: the training backward reads them)
 #pragma unroll
     for (int i = 0; i < GIN_PER; i++) {
       const int p = tid + i * 256;
@@ -771,6 +773,14 @@ __global__ __launch_bounds__(256, 2) void conv_first_pair_kernel(const ConvArgs 
       if (p < NA) {
         *(bf16x8 *)(s_a + p * 8) = v0;
         *(bf16x8 *)(s_a + PLANE_A + p * 8) = v1;
+        const int ry = p / AW, rx = p % AW;
+        if (a0.rec_out && ((valid >> i) & 1u) && ry >= 2 && ry < 2 + TH && rx >= 2 && rx < 2 + TW) {
+          bf16 *dst = a0.rec_out + (((size_t)n * a0.h + (y0 - 2 + ry)) * a0.w + (x0 - 2 + rx)) * 32;
+          *(bf16x8 *)dst = v0;
+          *(bf16x8 *)(dst + 8) = v1;
+          *(bf16x8 *)(dst + 16) = zero8;
+          *(bf16x8 *)(dst + 24) = zero8;
+        }
       }
     }
     __syncthreads();

@@ -146,7 +146,8 @@ int lgx_gather(const lgx_gather_params *p, void *workspace, const lgx_gbuffer *g
  * exactly what lgx_gather computes, with the workspace tables held in a
  * static device array of LGX_GATHER_SCRATCH_BYTES per GPU (kept until the
  * process ends) instead of caller memory.  Rejects parameters whose tables
- * exceed it with LGX_EUNSUPPORTED (only large FP64 table sets do).  The
+ * exceed it with LGX_EUNSUPPORTED (only large FP64 table sets do), and the
+ * swizzling launch shapes (their level copies would not fit).  The
  * first call with a given set of table parameters (extent, dirs, max_steps,
  * pattern, footprint, FP64 or not) on a device uploads the tables
  * synchronously on `stream`; later calls with the same set launch only the

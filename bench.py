@@ -41,6 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per unit (storage precision, no cache credit)
 BD, BR, BC, BO = 4, 8, 8, 16
 DEN_DIRS, TARGET_DIRS = 4, 256
+# a second denoised configuration, for the error bar: more directions on the
+# 4x4 pattern through the same network
+QDEN_DIRS, QDEN_GRID = 16, 4
 PLAIN_SWEEP = (32, 64, 128, 192, 256)
 QUALITY_SEEDS = (200, 201, 202, 203, 204, 205, 206, 207)
 BATCH_SEEDS = range(3000, 3032)
@@ -409,7 +412,15 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
             ms, _, _ = time_renderer(r, g, view, proj, max(3, args.steps // 4), 1)
             plain[f"{kind}{d}"] = {"renderer": r, "ms": ms, "err": []}
             log(f"{kind}{d}: {ms:.3f} ms/frame")
-    den_err, noisy_err = [], []
+    # the quality configuration: the same network on a 16-direction 4x4 input
+    q_tile, _ = dev.autotune_gather(g, pyr0, view, proj, QDEN_DIRS, dn.NOISY_STEPS, sampler, dn.NOISY_FOOTPRINT,
+                                    grid=QDEN_GRID)
+    rq = dev.Renderer(W, H, QDEN_DIRS, dn.NOISY_STEPS, LEVELS, sampler, QDEN_GRID, denoiser=lgx.DENOISER_UNET,
+                      kind="mixed", profile=True, footprint_dirs=dn.NOISY_FOOTPRINT, tile=q_tile)
+    rq.attach_unet(net)
+    q_ms, q_passes, _ = time_renderer(rq, g, view, proj, max(3, args.steps // 4), 1)
+    log(f"quality configuration ({QDEN_DIRS} directions, {QDEN_GRID}x{QDEN_GRID}): {q_ms:.3f} ms/frame")
+    den_err, noisy_err, q_err = [], [], []
     gs = dev.GBufferDev.alloc(W, H, "mixed")
     target = dev.Gather(W, H, TARGET_DIRS, 0, sampler, precision=lgx.PREC_F64)
     for s in seeds:
@@ -421,6 +432,8 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         rden.render(gs, v, p)
         den_err.append(rel_l2(rden.output(), tgt))
         noisy_err.append(rel_l2(rden.intermediate("indirect"), tgt))
+        rq.render(gs, v, p)
+        q_err.append(rel_l2(rq.output(), tgt))
         for e in plain.values():
             e["renderer"].render(gs, v, p)
             e["err"].append(rel_l2(e["renderer"].output(), tgt))
@@ -437,8 +450,15 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         if f is not None:
             fam[kind] = f
     eq_ms = min(fam.values()) if fam else None
+    qerr = float(np.mean(q_err))
+    q_fam = [equal_error_ms(qerr, [(float(np.mean(plain[f"{kind}{d}"]["err"])), plain[f"{kind}{d}"]["ms"])
+                                   for d in PLAIN_SWEEP]) for kind in ("plain", "raw")]
+    q_fam = [f for f in q_fam if f is not None]
+    q_eq = min(q_fam) if q_fam else None
+    log(f"quality configuration: rel-L2 {qerr:.5f}")
     for e in plain.values():
         e["renderer"].close()
+    rq.close()
     gx, gy = lgx.pattern(dn.NOISY_GRID)
     return {
         "workload": f"renderer 1920x1080: {DEN_DIRS}-direction gather (step growth and lod of {dn.NOISY_FOOTPRINT} "
@@ -458,6 +478,17 @@ def denoised_leg(args, g, view, proj, sampler, plain32_ms, seeds):
         "speedup_at_equal_error": round(eq_ms / den_ms, 3) if eq_ms else None,
         "speedup_vs_plain32": round(plain32_ms / den_ms, 3),
         "weights": os.path.relpath(path, ROOT),
+        "quality_config": {
+            "workload": f"the same network on a {QDEN_DIRS}-direction {QDEN_GRID}x{QDEN_GRID}-pattern gather "
+                        f"(footprint {dn.NOISY_FOOTPRINT}, no cap)",
+            "ms_per_frame": round(q_ms, 4),
+            "passes_ms": {f"{i}:{p['name']}": round(p["ms"], 4) for i, p in enumerate(q_passes)},
+            "gather_tile": q_tile,
+            "rel_l2_vs_fp64_256": round(qerr, 5),
+            "rel_l2_per_seed": [round(x, 5) for x in q_err],
+            "plain_ms_at_equal_error": round(q_eq, 4) if q_eq else None,
+            "speedup_at_equal_error": round(q_eq / q_ms, 3) if q_eq else None,
+        },
     }
 
 

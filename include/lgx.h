@@ -357,8 +357,10 @@ int lgx_gather_tile_info(int32_t tile, int32_t *info);
  * persistent kernel whose workgroups walk the views' tiles.  Each item is
  * what lgx_gather takes, checked the same way; all share LGX_PREC_F32, the
  * sampler, the tile and the radiance / normal formats (RGBA16F + RG16F or
- * RGBA32F + RG32F); step_counts must be NULL.  The descriptors and cameras
- * are uploaded at creation (synchronous); the images must stay allocated. */
+ * RGBA32F + RG32F); step_counts must be NULL; at most 2978 views (the
+ * swizzled copies' re-layout launch has one grid row per view level).  The
+ * descriptors and cameras are uploaded at creation (synchronous); the images
+ * must stay allocated. */
 typedef struct lgx_gather_batch lgx_gather_batch;
 int lgx_gather_batch_create(const lgx_gather_params *p, const void *const *workspaces, const lgx_gbuffer *g,
                             const lgx_pyramid *pyr, lgx_image *out, int32_t n, lgx_gather_batch **res);

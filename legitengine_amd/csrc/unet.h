@@ -96,8 +96,11 @@ hipError_t upsample_bwd(const bf16 *dfull, const bf16 *act, int n, int hh, int w
 // out row pitch in pixels: rows of w pixels (over all n images) start every out_pitch
 hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const float *bh, size_t npix, int w,
                     int out_pitch, float4 *out, hipStream_t s);
-hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
-                    float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s);
+// the head's forward, the loss and the head's backward in one pass (training;
+// the head's output is not stored)
+hipError_t head_train(const bf16 *d1, const float *x12, const float4 *target, const float *wh, const float *bh,
+                      size_t npix, float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss,
+                      hipStream_t s);
 // The convolutions' weights inside the flat parameter vector and their BF16
 // packed copies (pack_weights' layouts), for adam_pack.
 struct PackLayer {

@@ -322,7 +322,10 @@ class Net {
     check(conv_first_pair(c0, c1, s), "conv_first_pair");
   }
 
-  void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0) {
+  // head: also the 1x1 head into out_ (the training step forms the head in
+  // its backward pass instead)
+  void forward(const float *x12, int n, int h, int w, float *out_, hipStream_t s, int out_pitch = 0,
+               bool head = true) {
     const size_t P = (size_t)n * h * w;
     ConvArgs c0{};
     c0.gin_fmt = 3;
@@ -330,6 +333,7 @@ class Net {
     c0.rec_out = x32;  // the packed records: the first layer's weight gradient reads them
     first_pair(c0, n, h, w, e1a, s);
     body(n, h, w, s, 10, 2);
+    if (!head) return;
     check(head_fwd(d1, x12, params + head_w_off, params + head_b_off, P, w, out_pitch > 0 ? out_pitch : w,
                    (float4 *)out_, s),
           "head_fwd");
@@ -442,13 +446,13 @@ class Net {
   void backward(const float *x12, const float *target, int n, int h, int w, hipStream_t s) {
     const size_t P = (size_t)n * h * w;
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
-    forward(x12, n, h, w, out, s);
+    forward(x12, n, h, w, out, s, 0, false);
     // the head's gradient and the loss are accumulated atomically; every
     // convolution gradient is overwritten by its reduction
     check(fill(grads + head_w_off, (size_t)kHeadIn * kHeadOut + kHeadOut + 1, 0.f, s), "fill");  // and the loss
-    check(head_bwd(d1, (const float4 *)out, (const float4 *)target, params + head_w_off, P, 1.0f / (3.0f * P),
-                   loss_kind == LGX_UNET_LOSS_L2, g_d1, grads + head_w_off, grads + head_b_off, loss, s),
-          "head_bwd");
+    check(head_train(d1, x12, (const float4 *)target, params + head_w_off, params + head_b_off, P, 1.0f / (3.0f * P),
+                     loss_kind == LGX_UNET_LOSS_L2, g_d1, grads + head_w_off, grads + head_b_off, loss, s),
+          "head_train");
     // d1 = relu(conv9(d1a))
     wgrad(9, d1a, 32, 0, nullptr, 0, g_d1, n, h, w, s);
     dx_layer(9, n, h, w, s);

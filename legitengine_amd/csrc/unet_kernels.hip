@@ -1292,87 +1292,84 @@ __global__ __launch_bounds__(256) void head_fwd_img_kernel(const bf16 *d1, const
   out[(size_t)y * out_pitch + x] = make_float4(o[0], o[1], o[2], 1.f);
 }
 
-// Loss head backward.  Mean L1: g = sign(out - target) / count per channel;
-// mean squared error (l2): g = 2 (out - target) / count.  dd1 = mask(d1 > 0)
-// W_h^T g.  Four threads share a pixel, each owning 8 of its 32 channels
-// (one 16-byte load and store), so a thread keeps 3 x 8 head-weight partial
-// sums; they are reduced over the lanes of a wave with shuffles, over the
-// waves in LDS, and added to dW_h, db_h once per workgroup.
-__global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *__restrict__ d1, const float4 *__restrict__ out,
-                                                       const float4 *__restrict__ target, const float *__restrict__ wh,
-                                                       size_t npix, float inv_count, int l2, bf16 *__restrict__ dd1,
-                                                       float *dwh, float *dbh, float *loss) {
+// Training: the 1x1 head, the loss and the head's backward in one pass (the
+// head's output is not stored).  One pixel per thread: out = x + b_h + W_h
+// act, summed in head_fwd_kernel's order; mean L1: g = sign(out - target) /
+// count per channel, mean squared error (l2): g = 2 (out - target) / count;
+// dd1 = mask(d1 > 0) W_h^T g (per channel, the three products in channel
+// order).  A thread keeps 3 x 32 head-weight partial sums; they are reduced
+// over the lanes of a wave with shuffles, over the waves in LDS, and added to
+// dW_h, db_h and the loss once per workgroup.
+__global__ __launch_bounds__(256) void head_train_kernel(const bf16 *__restrict__ d1, const float *__restrict__ x12,
+                                                         const float4 *__restrict__ target, const float *__restrict__ wh,
+                                                         const float *__restrict__ bh, size_t npix, float inv_count,
+                                                         int l2, bf16 *__restrict__ dd1, float *dwh, float *dbh,
+                                                         float *loss) {
   __shared__ float red[4][100];
-  const int q = threadIdx.x & 3, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float w8[3][8], aw[3][8], ab[3] = {0.f, 0.f, 0.f}, al = 0.f;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float aw[3][32], ab[3] = {0.f, 0.f, 0.f}, al = 0.f;
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      w8[c][i] = wh[c * 32 + q * 8 + i];
-      aw[c][i] = 0.f;
-    }
-  // PB pixels per thread and pass, their loads issued together (pixels of a
-  // pass are a grid stride apart; the sums run over them in that order)
-  constexpr int PB = 4;
-  const size_t stride = (size_t)gridDim.x * 64;
-  for (size_t p0 = ((size_t)blockIdx.x * 256 + threadIdx.x) >> 2; p0 < npix; p0 += PB * stride) {
-    float4 o[PB], t[PB];
-    bf16x8 v[PB];
+    for (int i = 0; i < 32; i++) aw[c][i] = 0.f;
+  for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < npix; p += (size_t)gridDim.x * 256) {
+    bf16x8 v[4];
 #pragma unroll
-    for (int k = 0; k < PB; k++) {
-      const size_t p = min(p0 + k * stride, npix - 1);  // in bounds; pixels past the end are not used
-      o[k] = out[p];
-      t[k] = target[p];
-      v[k] = *(const bf16x8 *)(d1 + p * 32 + q * 8);
-    }
+    for (int q = 0; q < 4; q++) v[q] = *(const bf16x8 *)(d1 + p * 32 + q * 8);
+    const float4 t = target[p];
+    float o[3];
 #pragma unroll
-    for (int k = 0; k < PB; k++) {
-      const size_t p = p0 + k * stride;
-      if (p >= npix) break;
-      const float df[3] = {o[k].x - t[k].x, o[k].y - t[k].y, o[k].z - t[k].z};
-      float g[3];
+    for (int c = 0; c < 3; c++) o[c] = x12[p * 12 + c] + bh[c];
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        g[c] = (l2 ? 2.f * df[c] : (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f))) * inv_count;
-        al += l2 ? df[c] * df[c] : fabsf(df[c]);
-        ab[c] += g[c];
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float f = bf2f(v[q][i]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[c] += wh[c * 32 + q * 8 + i] * f;
       }
+    const float df[3] = {o[0] - t.x, o[1] - t.y, o[2] - t.z};
+    float g[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      g[c] = (l2 ? 2.f * df[c] : (df[c] > 0.f ? 1.f : (df[c] < 0.f ? -1.f : 0.f))) * inv_count;
+      al += l2 ? df[c] * df[c] : fabsf(df[c]);
+      ab[c] += g[c];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
       bf16x8 o8;
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const float f = bf2f(v[k][i]);
+        const float f = bf2f(v[q][i]);
         float sacc = 0.f;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          sacc += w8[c][i] * g[c];
-          aw[c][i] += g[c] * f;
+          sacc += wh[c * 32 + q * 8 + i] * g[c];
+          aw[c][q * 8 + i] += g[c] * f;
         }
         o8[i] = f > 0.f ? f2bf(sacc) : f2bf(0.f);
       }
       *(bf16x8 *)(dd1 + p * 32 + q * 8) = o8;
     }
   }
-  // lanes with the same q hold the same channels: reduce over lane bits 2..5
 #pragma unroll
-  for (int m = 4; m < 64; m <<= 1) {
+  for (int m = 1; m < 64; m <<= 1) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
 #pragma unroll
-      for (int i = 0; i < 8; i++) aw[c][i] += __shfl_xor(aw[c][i], m);
+      for (int i = 0; i < 32; i++) aw[c][i] += __shfl_xor(aw[c][i], m);
       ab[c] += __shfl_xor(ab[c], m);
     }
     al += __shfl_xor(al, m);
   }
-  if (lane < 4) {
+  if (lane == 0) {
 #pragma unroll
-    for (int c = 0; c < 3; c++)
+    for (int c = 0; c < 3; c++) {
 #pragma unroll
-      for (int i = 0; i < 8; i++) red[wave][c * 32 + q * 8 + i] = aw[c][i];
-  }
-  if (lane == 0) {  // every q saw the same pixels: the q = 0 lanes' sums count once
-#pragma unroll
-    for (int c = 0; c < 3; c++) red[wave][96 + c] = ab[c];
+      for (int i = 0; i < 32; i++) red[wave][c * 32 + i] = aw[c][i];
+      red[wave][96 + c] = ab[c];
+    }
     red[wave][99] = al;
   }
   __syncthreads();
@@ -1742,12 +1739,13 @@ hipError_t head_fwd(const bf16 *d1, const float *x12, const float *wh, const flo
                      out_pitch, out);
   return hipGetLastError();
 }
-hipError_t head_bwd(const bf16 *d1, const float4 *out, const float4 *target, const float *wh, size_t npix,
-                    float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss, hipStream_t s) {
-  // 64 pixels per workgroup pass; up to 512 workgroups, so 512 atomic sums per head parameter
-  const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(512, (npix + 63) / 64));
-  hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks), dim3(256), 0, s, d1, out, target, wh, npix, inv_count, l2, dd1,
-                     dwh, dbh, loss);
+hipError_t head_train(const bf16 *d1, const float *x12, const float4 *target, const float *wh, const float *bh,
+                      size_t npix, float inv_count, int l2, bf16 *dd1, float *dwh, float *dbh, float *loss,
+                      hipStream_t s) {
+  // persistent: about two workgroups per CU, so as many atomic sums per head parameter
+  const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(2 * cu_count(), (npix + 255) / 256));
+  hipLaunchKernelGGL(head_train_kernel, dim3(blocks), dim3(256), 0, s, d1, x12, target, wh, bh, npix, inv_count, l2,
+                     dd1, dwh, dbh, loss);
   return hipGetLastError();
 }
 hipError_t adam_pack(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,

@@ -1209,23 +1209,24 @@ __global__ __launch_bounds__(256) void avgpool_kernel(const bf16 *x, int n, int 
 
 // dx[p] = mask(act[p] > 0) * (dx_in[p] + dpool[p/2] / 4): the gradient of a
 // tensor feeding both a concatenation (dx_in) and an average pool (dpool).
+// Rows (image nn, row yy) over blockIdx.y, their pixels x 8-channel groups
+// over the threads: 32-bit index arithmetic, no division in the loop (c / 8
+// is a power of two).
 __global__ __launch_bounds__(256) void pool_bwd_add_kernel(const bf16 *dx_in, const bf16 *dpool, const bf16 *act, int n,
                                                            int h, int w, int c, bf16 *dx) {
-  const size_t total = (size_t)n * h * w * (c / 8);
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
-    const int q = (int)(e % (c / 8));
-    size_t p = e / (c / 8);
-    const int xx = (int)(p % w);
-    p /= w;
-    const int yy = (int)(p % h);
-    const int nn = (int)(p / h);
-    const size_t i0 = (((size_t)nn * h + yy) * w + xx) * c + q * 8;
-    const size_t ip = (((size_t)nn * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1)) * c + q * 8;
+  const int cq = c / 8, cs = __builtin_ctz((unsigned)cq);
+  for (int row = blockIdx.y; row < n * h; row += gridDim.y) {
+  const int yy = row % h, nn = row / h;
+  const size_t r0 = (size_t)row * w * c, rp = ((size_t)nn * (h >> 1) + (yy >> 1)) * (w >> 1) * c;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < w * cq; e += gridDim.x * 256) {
+    const int q = e & (cq - 1), xx = e >> cs;
+    const size_t i0 = r0 + (size_t)xx * c + q * 8, ip = rp + (size_t)(xx >> 1) * c + q * 8;
     const bf16x8 a = *(const bf16x8 *)(dx_in + i0), b = *(const bf16x8 *)(dpool + ip), m = *(const bf16x8 *)(act + i0);
     bf16x8 o;
 #pragma unroll
     for (int i = 0; i < 8; i++) o[i] = bf2f(m[i]) > 0.f ? f2bf(bf2f(a[i]) + 0.25f * bf2f(b[i])) : f2bf(0.f);
     *(bf16x8 *)(dx + i0) = o;
+  }
   }
 }
 
@@ -1724,8 +1725,10 @@ hipError_t avgpool(const bf16 *x, int n, int h, int w, int c, bf16 *y, hipStream
 }
 hipError_t pool_bwd_add(const bf16 *dx_in, const bf16 *dpool, const bf16 *act, int n, int h, int w, int c, bf16 *dx,
                         hipStream_t s) {
-  hipLaunchKernelGGL(pool_bwd_add_kernel, dim3(grid_for((size_t)n * h * w * (c / 8), 256)), dim3(256), 0, s, dx_in,
-                     dpool, act, n, h, w, c, dx);
+  if (c % 8 || (c / 8) & (c / 8 - 1) || h < 2 || w < 2) return hipErrorInvalidValue;
+  const unsigned bx = (unsigned)std::min((w * (c / 8) + 255) / 256, 64);
+  hipLaunchKernelGGL(pool_bwd_add_kernel, dim3(bx, (unsigned)std::min(n * h, 65535)), dim3(256), 0, s, dx_in, dpool,
+                     act, n, h, w, c, dx);
   return hipGetLastError();
 }
 hipError_t upsample_bwd(const bf16 *dfull, const bf16 *act, int n, int hh, int ww, int c, bf16 *dhalf, hipStream_t s) {
